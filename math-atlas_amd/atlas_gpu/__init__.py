@@ -1,0 +1,279 @@
+"""Python binding of libatlas_gpu.so (the MI355X FP64 level-3 library).
+
+Two layers, both straight ctypes calls into the C ABI declared in
+include/*.h:
+
+* the drop-in host-pointer entry points (cblas_dgemm, clapack_dgetrf, ...),
+  which take numpy arrays and behave like ATLAS's (synchronous, in place);
+* the device-pointer atlg_* API, which takes torch CUDA tensors holding
+  column-major matrices and queues work on a HIP stream.
+
+There is no fallback: if the shared library is missing, loading raises.
+Column-major matrices in torch are stored as a (cols, ld) row-major tensor,
+so element (i, j) is t[j, i]; `colmajor` and `from_fortran` build them.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+PKG_DIR = os.path.dirname(_HERE)
+LIB_PATH = os.path.join(PKG_DIR, "libatlas_gpu.so")
+
+RowMajor, ColMajor = 101, 102
+NoTrans, Trans, ConjTrans = 111, 112, 113
+Upper, Lower = 121, 122
+NonUnit, Unit = 131, 132
+Left, Right = 141, 142
+EHIP = -1000
+
+_dp = ctypes.POINTER(ctypes.c_double)
+_ip = ctypes.POINTER(ctypes.c_int)
+_i, _l, _d, _v = ctypes.c_int, ctypes.c_long, ctypes.c_double, ctypes.c_void_p
+_ul = ctypes.c_ulong
+
+XERBLA_HOOK = ctypes.CFUNCTYPE(None, ctypes.c_int, ctypes.c_char_p, ctypes.c_char_p)
+
+_SIGS = {
+    # drop-in host API
+    "cblas_dgemm": (None, [_i, _i, _i, _i, _i, _i, _d, _dp, _i, _dp, _i, _d, _dp, _i]),
+    "cblas_dtrsm": (None, [_i, _i, _i, _i, _i, _i, _i, _d, _dp, _i, _dp, _i]),
+    "cblas_dsyrk": (None, [_i, _i, _i, _i, _i, _d, _dp, _i, _d, _dp, _i]),
+    "clapack_dgetrf": (_i, [_i, _i, _i, _dp, _i, _ip]),
+    "clapack_dgetrs": (_i, [_i, _i, _i, _i, _dp, _i, _ip, _dp, _i]),
+    "clapack_dgesv": (_i, [_i, _i, _i, _dp, _i, _ip, _dp, _i]),
+    "clapack_dpotrf": (_i, [_i, _i, _i, _dp, _i]),
+    "clapack_dpotrs": (_i, [_i, _i, _i, _i, _dp, _i, _dp, _i]),
+    "clapack_dposv": (_i, [_i, _i, _i, _i, _dp, _i, _dp, _i]),
+    "ATL_dgemm": (None, [_i, _i, _ul, _ul, _ul, _d, _dp, _ul, _dp, _ul, _d, _dp, _ul]),
+    "ATL_dsyrk": (None, [_i, _i, _ul, _ul, _d, _dp, _ul, _d, _dp, _ul]),
+    "ATL_dtrsm": (None, [_i, _i, _i, _i, _ul, _ul, _d, _dp, _ul, _dp, _ul]),
+    "ATL_dgetrf": (_i, [_i, _i, _i, _dp, _i, _ip]),
+    "ATL_dgetrfC": (_i, [_i, _i, _dp, _i, _ip]),
+    "ATL_dgetrfR": (_i, [_i, _i, _dp, _i, _ip]),
+    "ATL_dgetrs": (None, [_i, _i, _i, _i, _dp, _i, _ip, _dp, _i]),
+    "ATL_dlaswp": (None, [_i, _dp, _i, _i, _i, _ip, _i]),
+    "ATL_dpotrf": (_i, [_i, _i, _i, _dp, _i]),
+    "ATL_dpotrfL": (_i, [_i, _dp, _i]),
+    "ATL_dpotrfU": (_i, [_i, _dp, _i]),
+    "ATL_dpotrs": (None, [_i, _i, _i, _i, _dp, _i, _dp, _i]),
+    # device API
+    "atlg_init": (_i, []),
+    "atlg_last_error": (ctypes.c_char_p, []),
+    "atlg_set_xerbla_hook": (None, [XERBLA_HOOK]),
+    "atlg_dgemm": (_i, [_i, _i, _l, _l, _l, _d, _v, _l, _v, _l, _d, _v, _l, _v]),
+    "atlg_dsyrk": (_i, [_i, _i, _l, _l, _d, _v, _l, _d, _v, _l, _v]),
+    "atlg_dtrsm": (_i, [_i, _i, _i, _i, _l, _l, _d, _v, _l, _v, _l, _v]),
+    "atlg_dlaswp": (_i, [_l, _v, _l, _i, _i, _v, _i, _v]),
+    "atlg_dgetrf": (_i, [_l, _l, _v, _l, _v, _v, _v]),
+    "atlg_dgetrs": (_i, [_i, _l, _l, _v, _l, _v, _v, _l, _v]),
+    "atlg_dpotrf": (_i, [_i, _l, _v, _l, _v, _v]),
+    "atlg_dpotrs": (_i, [_i, _l, _l, _v, _l, _v, _l, _v]),
+}
+
+_LIB = None
+
+
+def lib():
+    """Loads libatlas_gpu.so (raises if it has not been built)."""
+    global _LIB
+    if _LIB is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError(
+                "libatlas_gpu.so is not built (%s); run __graft_entry__.build()" % LIB_PATH)
+        L = ctypes.CDLL(LIB_PATH)
+        for name, (res, args) in _SIGS.items():
+            f = getattr(L, name)
+            f.restype = res
+            f.argtypes = args
+        _LIB = L
+    return _LIB
+
+
+def exported_names():
+    return list(_SIGS)
+
+
+class AtlgError(RuntimeError):
+    pass
+
+
+def _check(rc, what):
+    if rc == EHIP:
+        raise AtlgError("%s: HIP failure: %s" % (what, lib().atlg_last_error().decode()))
+    if rc != 0:
+        raise AtlgError("%s: illegal argument %d" % (what, -rc))
+
+
+# ---------------------------------------------------------------- xerbla hook
+
+class XerblaTrap:
+    """Context manager: records cblas_xerbla calls instead of exiting."""
+
+    def __init__(self):
+        self.calls = []
+        self._cb = XERBLA_HOOK(self._hook)
+
+    def _hook(self, p, rout, msg):
+        self.calls.append((p, rout.decode(), msg.decode()))
+
+    def __enter__(self):
+        lib().atlg_set_xerbla_hook(self._cb)
+        return self
+
+    def __exit__(self, *exc):
+        lib().atlg_set_xerbla_hook(XERBLA_HOOK())
+        return False
+
+
+# ---------------------------------------------------------------- host API
+
+def _p(a):
+    if a is None:
+        return ctypes.cast(None, _dp)
+    assert a.dtype == np.float64
+    return a.ctypes.data_as(_dp)
+
+
+def _pi(a):
+    assert a.dtype == np.int32
+    return a.ctypes.data_as(_ip)
+
+
+def cblas_dgemm(order, ta, tb, m, n, k, alpha, a, lda, b, ldb, beta, c, ldc):
+    lib().cblas_dgemm(order, ta, tb, m, n, k, alpha, _p(a), lda, _p(b), ldb, beta, _p(c), ldc)
+
+
+def cblas_dtrsm(order, side, uplo, ta, diag, m, n, alpha, a, lda, b, ldb):
+    lib().cblas_dtrsm(order, side, uplo, ta, diag, m, n, alpha, _p(a), lda, _p(b), ldb)
+
+
+def cblas_dsyrk(order, uplo, trans, n, k, alpha, a, lda, beta, c, ldc):
+    lib().cblas_dsyrk(order, uplo, trans, n, k, alpha, _p(a), lda, beta, _p(c), ldc)
+
+
+def clapack_dgetrf(order, m, n, a, lda, ipiv):
+    return lib().clapack_dgetrf(order, m, n, _p(a), lda, _pi(ipiv))
+
+
+def clapack_dgetrs(order, trans, n, nrhs, a, lda, ipiv, b, ldb):
+    return lib().clapack_dgetrs(order, trans, n, nrhs, _p(a), lda, _pi(ipiv), _p(b), ldb)
+
+
+def clapack_dgesv(order, n, nrhs, a, lda, ipiv, b, ldb):
+    return lib().clapack_dgesv(order, n, nrhs, _p(a), lda, _pi(ipiv), _p(b), ldb)
+
+
+def clapack_dpotrf(order, uplo, n, a, lda):
+    return lib().clapack_dpotrf(order, uplo, n, _p(a), lda)
+
+
+def clapack_dpotrs(order, uplo, n, nrhs, a, lda, b, ldb):
+    return lib().clapack_dpotrs(order, uplo, n, nrhs, _p(a), lda, _p(b), ldb)
+
+
+def clapack_dposv(order, uplo, n, nrhs, a, lda, b, ldb):
+    return lib().clapack_dposv(order, uplo, n, nrhs, _p(a), lda, _p(b), ldb)
+
+
+def ATL_dlaswp(n, a, lda, k1, k2, ipiv, inci):
+    lib().ATL_dlaswp(n, _p(a), lda, k1, k2, _pi(ipiv), inci)
+
+
+def ATL_dgemm(ta, tb, m, n, k, alpha, a, lda, b, ldb, beta, c, ldc):
+    lib().ATL_dgemm(ta, tb, m, n, k, alpha, _p(a), lda, _p(b), ldb, beta, _p(c), ldc)
+
+
+def ATL_dtrsm(side, uplo, ta, diag, m, n, alpha, a, lda, b, ldb):
+    lib().ATL_dtrsm(side, uplo, ta, diag, m, n, alpha, _p(a), lda, _p(b), ldb)
+
+
+def ATL_dgetrf(order, m, n, a, lda, ipiv):
+    return lib().ATL_dgetrf(order, m, n, _p(a), lda, _pi(ipiv))
+
+
+def ATL_dpotrf(order, uplo, n, a, lda):
+    return lib().ATL_dpotrf(order, uplo, n, _p(a), lda)
+
+
+# ---------------------------------------------------------------- device API
+
+def _torch():
+    import torch
+    if not torch.cuda.is_available():
+        raise RuntimeError("atlas_gpu device API needs a GPU (torch.cuda unavailable)")
+    return torch
+
+
+def colmajor(rows, cols, ld=None, device="cuda", fill=None):
+    """A column-major rows x cols FP64 matrix with leading dimension ld,
+    stored as a (cols, ld) torch tensor."""
+    torch = _torch()
+    ld = max(1, rows if ld is None else ld)
+    t = torch.empty((max(cols, 1), ld), dtype=torch.float64, device=device)
+    if fill is not None:
+        t.fill_(fill)
+    return t
+
+
+def from_fortran(a, device="cuda"):
+    """numpy (ld, cols) Fortran-ordered float64 -> device (cols, ld) tensor."""
+    torch = _torch()
+    a = np.asfortranarray(a, dtype=np.float64)
+    return torch.from_numpy(np.ascontiguousarray(a.T)).to(device)
+
+
+def to_fortran(t):
+    """device (cols, ld) tensor -> numpy (ld, cols) Fortran-ordered array."""
+    return np.asfortranarray(t.detach().cpu().numpy().T)
+
+
+def _ptr(t):
+    return ctypes.c_void_p(0 if t is None else t.data_ptr())
+
+
+def _stream(stream):
+    torch = _torch()
+    if stream is None:
+        stream = torch.cuda.current_stream()
+    return ctypes.c_void_p(stream.cuda_stream)
+
+
+def dgemm(ta, tb, m, n, k, alpha, a, lda, b, ldb, beta, c, ldc, stream=None):
+    _check(lib().atlg_dgemm(ta, tb, m, n, k, alpha, _ptr(a), lda, _ptr(b), ldb, beta, _ptr(c),
+                            ldc, _stream(stream)), "atlg_dgemm")
+
+
+def dsyrk(uplo, trans, n, k, alpha, a, lda, beta, c, ldc, stream=None):
+    _check(lib().atlg_dsyrk(uplo, trans, n, k, alpha, _ptr(a), lda, beta, _ptr(c), ldc,
+                            _stream(stream)), "atlg_dsyrk")
+
+
+def dtrsm(side, uplo, ta, diag, m, n, alpha, a, lda, b, ldb, stream=None):
+    _check(lib().atlg_dtrsm(side, uplo, ta, diag, m, n, alpha, _ptr(a), lda, _ptr(b), ldb,
+                            _stream(stream)), "atlg_dtrsm")
+
+
+def dlaswp(n, a, lda, k1, k2, ipiv, inci, stream=None):
+    _check(lib().atlg_dlaswp(n, _ptr(a), lda, k1, k2, _ptr(ipiv), inci, _stream(stream)),
+           "atlg_dlaswp")
+
+
+def dgetrf(m, n, a, lda, ipiv, info, stream=None):
+    _check(lib().atlg_dgetrf(m, n, _ptr(a), lda, _ptr(ipiv), _ptr(info), _stream(stream)),
+           "atlg_dgetrf")
+
+
+def dgetrs(trans, n, nrhs, a, lda, ipiv, b, ldb, stream=None):
+    _check(lib().atlg_dgetrs(trans, n, nrhs, _ptr(a), lda, _ptr(ipiv), _ptr(b), ldb,
+                             _stream(stream)), "atlg_dgetrs")
+
+
+def dpotrf(uplo, n, a, lda, info, stream=None):
+    _check(lib().atlg_dpotrf(uplo, n, _ptr(a), lda, _ptr(info), _stream(stream)), "atlg_dpotrf")
+
+
+def dpotrs(uplo, n, nrhs, a, lda, b, ldb, stream=None):
+    _check(lib().atlg_dpotrs(uplo, n, nrhs, _ptr(a), lda, _ptr(b), ldb, _stream(stream)),
+           "atlg_dpotrs")

@@ -1,0 +1,209 @@
+// Triangular solve with many right-hand sides on the FP64 matrix cores.
+//
+// Replaces ATL_dtrsm (AtlasBase/Clint/trsm-micro.base:200-420: recursive
+// halving trsm_recL/R, then iptrsm/trsmL_LNUT/utrsm with reciprocal
+// diagonals, :550-1700, and the nano-kernels of
+// AtlasBase/Students/trsm-nano.base:297-344).  ATLAS keeps reciprocal
+// diagonals and runs everything off the diagonal as amm GEMM; this version
+// goes one step further:
+//   1. one launch inverts every NB x NB diagonal block of op(A) in LDS
+//      (one workgroup per block), writing full NB x NB inverses;
+//   2. a recursive halving driver (the same split as trsm_recL/R) applies
+//      each inverse to its block row/column as an in-place GEMM and updates
+//      the rest with large GEMMs.
+// All flops but the small inversions run on v_mfma_f64_16x16x4_f64.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+
+#include "atlg_internal.hpp"
+
+namespace atlg {
+namespace {
+
+constexpr int NB = 128;        // diagonal block order (<= GEMM tile edge)
+constexpr int LDT = NB + 1;    // LDS row pitch
+constexpr int TRI_THREADS = 256;
+
+// Element (i, j) of op(A).
+__device__ inline double opa(const double* A, long lda, bool trans, long i, long j) {
+  return trans ? A[j + i * lda] : A[i + j * lda];
+}
+
+// Inverts the diagonal blocks of op(A) (order na).  Block b covers rows and
+// columns [b*NB, b*NB + nbb).  An upper-triangular op(A) block U is handled
+// as the lower-triangular J U J (J = order reversal), so one in-place lower
+// inversion serves all cases.  Output: Dinv + b*NB*NB, column-major, ld NB,
+// zeros outside the triangle; Unit diagonals are never read.
+__global__ void __launch_bounds__(TRI_THREADS)
+trtri_diag_kernel(int na, const double* __restrict__ A, long lda, int trans, int lower,
+                  int unit, double* __restrict__ Dinv, const int* __restrict__ skip) {
+  if (skip && *skip) return;
+  __shared__ double T[NB * LDT];
+  __shared__ double xs[NB];
+  const int b = blockIdx.x, t = threadIdx.x;
+  const long o = (long)b * NB;
+  const int n = (int)min((long)NB, na - o);
+  const bool tr = trans != 0;
+
+  // T[i][j] (stored T[i*LDT + j]) = lower-triangular form of the block.
+  for (int idx = t; idx < n * n; idx += TRI_THREADS) {
+    const int i = idx % n, j = idx / n;
+    double v = 0.0;
+    if (i >= j) {
+      if (lower)
+        v = (i == j && unit) ? 1.0 : opa(A, lda, tr, o + i, o + j);
+      else
+        v = (i == j && unit) ? 1.0 : opa(A, lda, tr, o + n - 1 - i, o + n - 1 - j);
+    }
+    T[i * LDT + j] = v;
+  }
+  __syncthreads();
+
+  // In-place inversion, last column first: for column j, with the trailing
+  // block already inverted, x <- -d * inv(T22) * T(j+1:n, j), d = 1/T(j,j).
+  for (int j = n - 1; j >= 0; j--) {
+    const double d = unit ? 1.0 : 1.0 / T[j * LDT + j];
+    for (int i = j + 1 + t; i < n; i += TRI_THREADS) xs[i] = T[i * LDT + j];
+    __syncthreads();
+    for (int i = j + 1 + t; i < n; i += TRI_THREADS) {
+      double y = 0.0;
+      for (int k = j + 1; k <= i; k++) y += T[i * LDT + k] * xs[k];
+      T[i * LDT + j] = -d * y;
+    }
+    if (t == 0) T[j * LDT + j] = d;
+    __syncthreads();
+  }
+
+  double* D = Dinv + (size_t)b * NB * NB;
+  for (int idx = t; idx < NB * NB; idx += TRI_THREADS) {
+    const int i = idx % NB, j = idx / NB;
+    double v = 0.0;
+    if (i < n && j < n) {
+      if (lower)
+        v = (i >= j) ? T[i * LDT + j] : 0.0;
+      else
+        v = (i <= j) ? T[(n - 1 - i) * LDT + (n - 1 - j)] : 0.0;
+    }
+    D[i + (size_t)j * NB] = v;
+  }
+}
+
+struct Ctx {
+  hipStream_t s;
+  bool left, lower_eff, trans;
+  const double* A;
+  long lda;
+  double* B;
+  long ldb;
+  long M, N;   // B is M x N
+  const double* Dinv;
+  const int* skip;
+};
+
+// op(A)[i0:, j0:] as a GEMM operand: pointer and transpose flag.
+inline void opa_block(const Ctx& c, long i0, long j0, const double** p, int* t) {
+  if (!c.trans) {
+    *p = c.A + i0 + j0 * c.lda;
+    *t = kNoTrans;
+  } else {
+    *p = c.A + j0 + i0 * c.lda;
+    *t = kTrans;
+  }
+}
+
+long split(long n) {
+  long n1 = ((n / 2) / NB) * NB;
+  return n1 > 0 ? n1 : NB;
+}
+
+// Solves on rows/cols [r0, r0+n) of op(A) with scale alpha.
+void solve(const Ctx& c, long r0, long n, double alpha) {
+  if (n <= NB) {
+    const double* D = c.Dinv + (size_t)(r0 / NB) * NB * NB;
+    if (c.left) {
+      double* Bb = c.B + r0;
+      gemm(c.s, kNoTrans, kNoTrans, n, c.N, n, alpha, D, NB, Bb, c.ldb, 0.0, Bb, c.ldb, 0,
+           c.skip);
+    } else {
+      double* Bb = c.B + r0 * c.ldb;
+      gemm(c.s, kNoTrans, kNoTrans, c.M, n, n, alpha, Bb, c.ldb, D, NB, 0.0, Bb, c.ldb, 0,
+           c.skip);
+    }
+    return;
+  }
+  const long n1 = split(n), n2 = n - n1;
+  const double* Ap;
+  int ta;
+  if (c.left) {
+    double* B1 = c.B + r0;
+    double* B2 = c.B + r0 + n1;
+    if (c.lower_eff) {
+      // [L11 0; L21 L22]: X1 first, then B2 <- alpha*B2 - L21*X1
+      solve(c, r0, n1, alpha);
+      opa_block(c, r0 + n1, r0, &Ap, &ta);
+      gemm(c.s, ta, kNoTrans, n2, c.N, n1, -1.0, Ap, c.lda, B1, c.ldb, alpha, B2, c.ldb, 0,
+           c.skip);
+      solve(c, r0 + n1, n2, 1.0);
+    } else {
+      // [U11 U12; 0 U22]: X2 first, then B1 <- alpha*B1 - U12*X2
+      solve(c, r0 + n1, n2, alpha);
+      opa_block(c, r0, r0 + n1, &Ap, &ta);
+      gemm(c.s, ta, kNoTrans, n1, c.N, n2, -1.0, Ap, c.lda, B2, c.ldb, alpha, B1, c.ldb, 0,
+           c.skip);
+      solve(c, r0, n1, 1.0);
+    }
+  } else {
+    double* B1 = c.B + r0 * c.ldb;
+    double* B2 = c.B + (r0 + n1) * c.ldb;
+    if (c.lower_eff) {
+      // X [L11 0; L21 L22] = B: X2 first, then B1 <- alpha*B1 - X2*L21
+      solve(c, r0 + n1, n2, alpha);
+      opa_block(c, r0 + n1, r0, &Ap, &ta);
+      gemm(c.s, kNoTrans, ta, c.M, n1, n2, -1.0, B2, c.ldb, Ap, c.lda, alpha, B1, c.ldb, 0,
+           c.skip);
+      solve(c, r0, n1, 1.0);
+    } else {
+      // X [U11 U12; 0 U22] = B: X1 first, then B2 <- alpha*B2 - X1*U12
+      solve(c, r0, n1, alpha);
+      opa_block(c, r0, r0 + n1, &Ap, &ta);
+      gemm(c.s, kNoTrans, ta, c.M, n2, n1, -1.0, B1, c.ldb, Ap, c.lda, alpha, B2, c.ldb, 0,
+           c.skip);
+      solve(c, r0 + n1, n2, 1.0);
+    }
+  }
+}
+
+}  // namespace
+
+void trsm(hipStream_t s, int side, int uplo, int ta, int diag, long M, long N, double alpha,
+          const double* A, long lda, double* B, long ldb, const int* skip) {
+  if (M <= 0 || N <= 0) return;
+  if (alpha == 0.0) {
+    scale(s, M, N, 0.0, B, ldb, 0, skip);
+    return;
+  }
+  Ctx c;
+  c.s = s;
+  c.left = (side == kLeft);
+  c.trans = is_trans(ta);
+  c.lower_eff = ((uplo == kLower) != c.trans);
+  c.A = A;
+  c.lda = lda;
+  c.B = B;
+  c.ldb = ldb;
+  c.M = M;
+  c.N = N;
+  c.skip = skip;
+  const long na = c.left ? M : N;
+  const long nblk = (na + NB - 1) / NB;
+  double* Dinv = static_cast<double*>(scratch(s, kSlotTrsmInv, sizeof(double) * nblk * NB * NB));
+  hipLaunchKernelGGL(trtri_diag_kernel, dim3((unsigned)nblk), dim3(TRI_THREADS), 0, s, (int)na,
+                     A, lda, c.trans ? 1 : 0, (uplo == kLower) != c.trans ? 1 : 0,
+                     diag == kUnit ? 1 : 0, Dinv, skip);
+  ATLG_LAUNCHED();
+  c.Dinv = Dinv;
+  solve(c, 0, na, alpha);
+}
+
+}  // namespace atlg
