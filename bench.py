@@ -1,0 +1,270 @@
+"""Benchmark of the FP64 level-3 hot path on one MI355X.
+
+Headline (one JSON line): DGEMM C = A*B + C, column-major FP64,
+M = N = K = 16384, alpha = beta = 1, A/B/C iid U[-1,1] (seeded, generated on
+the device) -- the N=16384 DGEMM of the FP64 level-3 sweep.  One "step" is
+one DGEMM through the C ABI (atlg_dgemm) on torch's current stream; inputs
+are resident in HBM before timing starts.
+
+Also reported in the same line (`sweep`): DGEMM N=4096/8192, tall-skinny
+M=65536 N=K=256, DTRSM L/L/N/U N=16384 with 16384 RHS, DGETRF and DPOTRF
+N=16384 (TFLOP/s with ATLAS's flop formulas), and the HBM rate of the laswp
+row interchanges inside DGETRF.
+
+`roofline` is computed from HIP events recorded on the stream the DGEMM
+kernels run on; `cpu_baseline` times the CPU oracle's textbook DGEMM on a
+bounded column slab of the same N=16384 problem.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+for p in (os.path.join(ROOT, "math-atlas_amd"), os.path.join(ROOT, "oracle")):
+    if p not in sys.path:
+        sys.path.insert(0, p)
+
+METRIC = "DGEMM TFLOP/s %FP64-peak; FP64-accurate solve time, IR-mixed vs plain dgesv"
+PEAK_FP64_MFMA = 78.6      # TFLOP/s, AMD published dense FP64 matrix peak
+PEAK_HBM = 8000.0          # GB/s
+
+
+def parse():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--n", type=int, default=16384)
+    ap.add_argument("--no-sweep", action="store_true")
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--cpu-cols", type=int, default=160)
+    ap.add_argument("--only", default="", help="profile helper: run only this sweep item")
+    ap.add_argument("--copy-calib", action="store_true",
+                    help="profile helper: three 1 GiB device-to-device copies (HBM counter scale)")
+    return ap.parse_args()
+
+
+def uniform(torch, shape, seed):
+    g = torch.Generator(device="cuda")
+    g.manual_seed(seed)
+    t = torch.empty(shape, dtype=torch.float64, device="cuda")
+    t.uniform_(-1.0, 1.0, generator=g)
+    return t
+
+
+def timed(torch, fn, reps, warm=1):
+    for _ in range(warm):
+        fn()
+    torch.cuda.synchronize()
+    best = None
+    for _ in range(reps):
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record()
+        fn()
+        e1.record()
+        e1.synchronize()
+        ms = e0.elapsed_time(e1)
+        best = ms if best is None else min(best, ms)
+    return best
+
+
+def sweep(torch, G, only=""):
+    out = {}
+
+    def want(k):
+        return not only or only == k
+
+    for n in (4096, 8192):
+        if not want("dgemm_%d" % n):
+            continue
+        a, b, c = (uniform(torch, (n, n), 10 + i) for i in range(3))
+        ms = timed(torch, lambda: G.dgemm(G.NoTrans, G.NoTrans, n, n, n, 1.0, a, n, b, n, 1.0, c, n), 3)
+        out["dgemm_%d_tflops" % n] = 2.0 * n ** 3 / ms / 1e9
+        del a, b, c
+    if want("dgemm_ts"):
+        m, n, k = 65536, 256, 256
+        a, b, c = uniform(torch, (k, m), 20), uniform(torch, (n, k), 21), uniform(torch, (n, m), 22)
+        ms = timed(torch, lambda: G.dgemm(G.NoTrans, G.NoTrans, m, n, k, 1.0, a, m, b, k, 1.0, c, m), 5)
+        out["dgemm_ts_65536x256x256_tflops"] = 2.0 * m * n * k / ms / 1e9
+        out["dgemm_ts_65536x256x256_GBps"] = 8.0 * (m * k + k * n + 2 * m * n) / ms / 1e6
+        del a, b, c
+    N = 16384
+    if want("dtrsm"):
+        a = uniform(torch, (N, N), 30)
+        a.mul_(1.0 / N)   # trddom: strict lower part scaled by 1/N (unit diagonal not read)
+        b = uniform(torch, (N, N), 31)
+        b0 = b.clone()
+
+        def run_trsm():
+            b.copy_(b0)
+            G.dtrsm(G.Left, G.Lower, G.NoTrans, G.Unit, N, N, 1.0, a, N, b, N)
+        ms_copy = timed(torch, lambda: b.copy_(b0), 2)
+        ms = timed(torch, run_trsm, 2) - ms_copy
+        out["dtrsm_LLNU_16384x16384_tflops"] = float(N) * N * N / ms / 1e9
+        del a, b, b0
+    if want("dgetrf"):
+        a0 = uniform(torch, (N, N), 40)
+        a = a0.clone()
+        ipiv = torch.zeros(N, dtype=torch.int32, device="cuda")
+        info = torch.zeros(1, dtype=torch.int32, device="cuda")
+
+        def run_lu():
+            a.copy_(a0)
+            G.dgetrf(N, N, a, N, ipiv, info)
+        ms_copy = timed(torch, lambda: a.copy_(a0), 2)
+        ms = timed(torch, run_lu, 1) - ms_copy
+        flops = 2.0 * N ** 3 / 3 - N ** 2 / 2.0 - N / 6.0
+        out["dgetrf_16384_tflops"] = flops / ms / 1e9
+        out["dgetrf_16384_ms"] = ms
+        out["dgetrf_16384_info"] = int(info.item())
+        del a, a0
+    if want("dpotrf"):
+        bm = torch.empty((N, N), dtype=torch.float64, device="cuda")
+        g = torch.Generator(device="cuda")
+        g.manual_seed(50)
+        bm.normal_(generator=g)
+        s0 = torch.mm(bm, bm.t()).div_(N)   # B B^T / N + N I (symmetric, SPD)
+        s0.diagonal().add_(float(N))
+        del bm
+        s = s0.clone()
+        info = torch.zeros(1, dtype=torch.int32, device="cuda")
+
+        def run_chol():
+            s.copy_(s0)
+            G.dpotrf(G.Lower, N, s, N, info)
+        ms_copy = timed(torch, lambda: s.copy_(s0), 2)
+        ms = timed(torch, run_chol, 2) - ms_copy
+        flops = N ** 3 / 3.0 + N ** 2 / 2.0 + N / 6.0
+        out["dpotrf_16384_tflops"] = flops / ms / 1e9
+        out["dpotrf_16384_ms"] = ms
+        out["dpotrf_16384_info"] = int(info.item())
+        del s, s0
+    torch.cuda.empty_cache()
+    return out
+
+
+def cpu_baseline(n, ncols):
+    import numpy as np
+    import oracle as O
+    rng = np.random.default_rng(7)
+    a = np.asfortranarray(rng.uniform(-1.0, 1.0, (n, n)))
+    b = np.asfortranarray(rng.uniform(-1.0, 1.0, (n, ncols)))
+    c = np.asfortranarray(rng.uniform(-1.0, 1.0, (n, ncols)))
+    sec = O.time_refgemm_slab(n, ncols, n, a, b, c)
+    flops = 2.0 * n * ncols * n
+    return {"value": flops / sec / 1e12, "unit": "TFLOP/s", "cores": 1, "kind": "port",
+            "sample": "oracle textbook DGEMM (ATL_drefgemmNN loop order), M=K=%d, first %d "
+                      "columns of C (%.1f GFLOP in %.1f s), 1 thread" % (n, ncols, flops / 1e9, sec)}
+
+
+def main():
+    args = parse()
+    if args.gpus != 1:
+        print("bench.py measures one GPU: this build runs the level-3 path on a single "
+              "MI355X; --gpus %d is not supported" % args.gpus, file=sys.stderr)
+        sys.exit(2)
+    import torch
+    import atlas_gpu as G
+
+    assert torch.cuda.is_available(), "bench.py needs an MI355X"
+    torch.cuda.set_device(0)
+    G.lib()
+    assert G.lib().atlg_init() == 0
+
+    if args.only:
+        print(json.dumps(sweep(torch, G, args.only)), flush=True)
+        return
+    if args.copy_calib:
+        src = torch.ones(1 << 27, dtype=torch.float64, device="cuda")
+        dst = torch.empty_like(src)
+        for _ in range(3):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_bytes_each": 2 * src.numel() * 8, "copies": 3}), flush=True)
+        return
+
+    n = args.n
+    a = uniform(torch, (n, n), 1)
+    b = uniform(torch, (n, n), 2)
+    c = uniform(torch, (n, n), 3)
+    stream = torch.cuda.current_stream()
+
+    def step():
+        G.dgemm(G.NoTrans, G.NoTrans, n, n, n, 1.0, a, n, b, n, 1.0, c, n, stream=stream)
+
+    for _ in range(args.warmup):
+        step()
+    torch.cuda.synchronize()
+    # per-launch events on the kernels' stream, plus the whole timed region
+    evs = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True))
+           for _ in range(args.steps)]
+    t0 = time.perf_counter()
+    for e0, e1 in evs:
+        e0.record(stream)
+        step()
+        e1.record(stream)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    per = [e0.elapsed_time(e1) for e0, e1 in evs]
+    ms_region = evs[0][0].elapsed_time(evs[-1][1])
+    ms_per_step = ms_region / args.steps
+    flops = 2.0 * n ** 3
+    value = flops / ms_per_step / 1e9
+    avg_launch = sum(per) / len(per)
+    achieved = flops / avg_launch / 1e9
+
+    # spot check of the result rows against torch's FP64 GEMM (checker only)
+    rows = torch.arange(0, n, 4099, device="cuda")
+    a2, b2, c2 = uniform(torch, (n, n), 1), uniform(torch, (n, n), 2), uniform(torch, (n, n), 3)
+    G.dgemm(G.NoTrans, G.NoTrans, n, n, n, 1.0, a2, n, b2, n, 1.0, c2, n, stream=stream)
+    # column-major X is stored as x (cols, ld): X = x^T, so C[rows] = A[rows] B + C0[rows]
+    a_rows = a2.t()[rows].contiguous()
+    bmat = b2.t().contiguous()
+    ref = torch.mm(a_rows, bmat) + uniform(torch, (n, n), 3).t()[rows]
+    got = c2.t()[rows]
+    rel = float((got - ref).abs().max() / ref.abs().max())
+    del a2, b2, c2, ref, got, a_rows, bmat
+
+    traffic = None
+    tf = os.path.join(ROOT, "profiles", "dgemm_n16384_traffic.json")
+    if os.path.exists(tf):
+        with open(tf) as f:
+            traffic = json.load(f).get("hbm_bytes_per_launch")
+
+    out = {
+        "metric": METRIC,
+        "value": round(value, 3),
+        "unit": "TFLOP/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms_per_step, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic",
+        "config": {"workload": "DGEMM NN column-major FP64 M=N=K=%d alpha=beta=1, U[-1,1]" % n,
+                   "M": n, "N": n, "K": n, "alpha": 1.0, "beta": 1.0},
+        "pct_fp64_peak": round(100.0 * value / PEAK_FP64_MFMA, 2),
+        "roofline": {"bound": "mfma", "achieved": round(achieved, 3), "peak": PEAK_FP64_MFMA,
+                     "unit": "TFLOP/s", "frac": round(achieved / PEAK_FP64_MFMA, 4),
+                     "traffic": traffic, "kernel": "dgemm_mfma<NN>",
+                     "avg_launch_ms": round(avg_launch, 4)},
+        "check_max_rel_err": rel,
+        "wall_s_timed": round(wall, 3),
+    }
+    del a, b, c
+    torch.cuda.empty_cache()
+    if not args.no_sweep:
+        out["sweep"] = {k: (round(v, 4) if isinstance(v, float) else v)
+                        for k, v in sweep(torch, G).items()}
+    if not args.no_cpu:
+        out["cpu_baseline"] = cpu_baseline(n, args.cpu_cols)
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -16,10 +16,11 @@
  * no call synchronizes.  ipiv arrays live in device memory as int32, 0-based
  * absolute row indices, as in ATLAS's C interface.
  *
- * Streams: `stream` is a hipStream_t passed as void*; NULL selects the
- * library's own stream on the current device.  Workspace is owned by the
+ * Streams: `stream` is a hipStream_t passed as void*; NULL is HIP's null
+ * (default) stream, as in other HIP libraries.  Workspace is owned by the
  * library per stream and reused across calls; calls on different streams
- * may run concurrently.
+ * may run concurrently.  The host-pointer entry points use a stream of the
+ * library's own.
  */
 #ifndef ATLAS_GPU_H
 #define ATLAS_GPU_H
@@ -31,7 +32,7 @@ extern "C" {
 
 typedef void *atlg_stream_t;
 
-/* Binds the library to the current HIP device (idempotent). */
+/* Creates the library's stream on the current HIP device (idempotent). */
 int atlg_init(void);
 /* Message of the last failed call on this thread (static storage). */
 const char *atlg_last_error(void);

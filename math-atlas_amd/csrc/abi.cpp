@@ -60,7 +60,7 @@ struct Stage {
   }
 };
 
-hipStream_t lib_stream() { return resolve_stream(nullptr); }
+hipStream_t lib_stream() { return library_stream(); }
 
 // ---- host drivers on column-major data ------------------------------------
 

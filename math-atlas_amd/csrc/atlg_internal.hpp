@@ -45,8 +45,11 @@ struct HipFailure {
                                hipGetErrorString(e_)};                         \
   } while (0)
 
-// Resolves NULL to the library stream of the current device.
+// The caller's stream (NULL is HIP's null stream).
 hipStream_t resolve_stream(atlg_stream_t s);
+// The library's own stream on the current device, used by the host-pointer
+// entry points (created on first use, default priority, blocking).
+hipStream_t library_stream();
 
 // Per-stream scratch memory in named slots.  A slot grows (after a stream
 // synchronize) and is reused by later calls on the same stream; pointers from

@@ -29,16 +29,18 @@ constexpr int TM = WM / 16, TN = WN / 16; // MFMA tiles per wave
 constexpr int PADA = 4, PADB = 4;         // LDS row padding (doubles)
 constexpr int LDA_S = BM + PADA, LDB_S = BN + PADB;
 
-// Register layout of v_mfma_f64_16x16x4_f64 (D = A*B + C, A 16x4, B 4x16):
-// lane l supplies A[row][k] and B[k][col]; D[i][j] lives in lane/reg below.
-// Measured on gfx950 with one-hot inputs.
+// Register layout of v_mfma_f64_16x16x4_f64 (D = A*B + C, A 16x4, B 4x16),
+// measured on gfx950 with one-hot operands: lane l supplies A[l&15][l>>4]
+// and B[l>>4][l&15]; D[i][j] is in lane j + 16*(i&3), register i>>2.
+// The kernel passes the B tile as the first operand and the A tile as the
+// second, so the accumulator holds C^T: lane l, register r carries
+// C[m = l&15][n = (l>>4) + 4r], and a register's 16 lanes of one row group
+// are 16 consecutive rows of a column (one 128-byte store).
 struct MfmaF64 {
-  __device__ static int a_row(int l) { return l & 15; }
-  __device__ static int a_k(int l) { return l >> 4; }
-  __device__ static int b_col(int l) { return l & 15; }
-  __device__ static int b_k(int l) { return l >> 4; }
-  __device__ static int d_row(int l, int r) { return 4 * (l >> 4) + r; }
-  __device__ static int d_col(int l, int /*r*/) { return l & 15; }
+  __device__ static int frag_x(int l) { return l & 15; }   // m (or n) of a fragment
+  __device__ static int frag_k(int l) { return l >> 4; }   // k of a fragment
+  __device__ static int c_m(int l) { return l & 15; }
+  __device__ static int c_n(int l, int r) { return (l >> 4) + 4 * r; }
 };
 
 // Loads one BK x 128 operand tile into registers.  Element (x, k) of the
@@ -128,7 +130,7 @@ __device__ inline void tile_of(int bid, int nblk, int tilesM, int tilesN, int& t
 }
 
 template <bool AT, bool BT, bool VA, bool VB>
-__global__ void __launch_bounds__(NT)
+__global__ void __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(2, 2)))
 dgemm_mfma(int M, int N, int K, double alpha, const double* A, long lda,
            const double* B, long ldb, double beta, double* C, long ldc, int mode,
            const int* __restrict__ skip) {
@@ -164,8 +166,7 @@ dgemm_mfma(int M, int N, int K, double alpha, const double* A, long lda,
   lb.template store<LDB_S>(Bs[0], t);
   __syncthreads();
 
-  const int ar = MfmaF64::a_row(lane), ak = MfmaF64::a_k(lane);
-  const int bc = MfmaF64::b_col(lane), bk = MfmaF64::b_k(lane);
+  const int fx = MfmaF64::frag_x(lane), fk = MfmaF64::frag_k(lane);
 
   for (int kt = 0; kt < nk; kt++) {
     const int cur = kt & 1;
@@ -180,14 +181,14 @@ dgemm_mfma(int M, int N, int K, double alpha, const double* A, long lda,
     for (int ks = 0; ks < BK / 4; ks++) {
       double af[TM], bf[TN];
 #pragma unroll
-      for (int i = 0; i < TM; i++) af[i] = as[(ks * 4 + ak) * LDA_S + wm0 + i * 16 + ar];
+      for (int i = 0; i < TM; i++) af[i] = as[(ks * 4 + fk) * LDA_S + wm0 + i * 16 + fx];
 #pragma unroll
-      for (int j = 0; j < TN; j++) bf[j] = bs[(ks * 4 + bk) * LDB_S + wn0 + j * 16 + bc];
+      for (int j = 0; j < TN; j++) bf[j] = bs[(ks * 4 + fk) * LDB_S + wn0 + j * 16 + fx];
 #pragma unroll
       for (int i = 0; i < TM; i++)
 #pragma unroll
         for (int j = 0; j < TN; j++)
-          acc[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(af[i], bf[j], acc[i][j], 0, 0, 0);
+          acc[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(bf[j], af[i], acc[i][j], 0, 0, 0);
     }
     if (more) {
       la.template store<LDA_S>(As[cur ^ 1], t);
@@ -204,8 +205,8 @@ dgemm_mfma(int M, int N, int K, double alpha, const double* A, long lda,
     for (int j = 0; j < TN; j++)
 #pragma unroll
       for (int r = 0; r < 4; r++) {
-        const int gi = m0 + wm0 + i * 16 + MfmaF64::d_row(lane, r);
-        const int gj = n0 + wn0 + j * 16 + MfmaF64::d_col(lane, r);
+        const int gi = m0 + wm0 + i * 16 + MfmaF64::c_m(lane);
+        const int gj = n0 + wn0 + j * 16 + MfmaF64::c_n(lane, r);
         if (gi >= M || gj >= N) continue;
         if (mode == 1 && gi < gj) continue;
         if (mode == 2 && gi > gj) continue;

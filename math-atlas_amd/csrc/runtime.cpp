@@ -33,15 +33,16 @@ std::map<std::pair<hipStream_t, int>, SlotBuf> g_slots;
 
 void set_error(const std::string& msg) { g_last_error = msg; }
 
-hipStream_t resolve_stream(atlg_stream_t s) {
-  if (s) return reinterpret_cast<hipStream_t>(s);
+hipStream_t resolve_stream(atlg_stream_t s) { return reinterpret_cast<hipStream_t>(s); }
+
+hipStream_t library_stream() {
   int dev = 0;
   ATLG_HIP(hipGetDevice(&dev));
   std::lock_guard<std::mutex> lk(g_mu);
   auto it = g_streams.find(dev);
   if (it != g_streams.end()) return it->second;
   hipStream_t st;
-  ATLG_HIP(hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
+  ATLG_HIP(hipStreamCreate(&st));
   g_streams[dev] = st;
   return st;
 }
@@ -101,7 +102,7 @@ bool bad_side(int s) { return s != kLeft && s != kRight; }
 extern "C" {
 
 int atlg_init(void) {
-  return guarded([] { (void)resolve_stream(nullptr); });
+  return guarded([] { (void)library_stream(); });
 }
 
 const char* atlg_last_error(void) { return g_last_error.c_str(); }

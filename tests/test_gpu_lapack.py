@@ -63,12 +63,13 @@ def test_getrf_panel_paths(G):
 
 
 def test_getrf_known_pivots(G):
-    # A = P^T L U with |L_ij| < 1/2: every pivot is the unit-diagonal row, at
-    # least twice the size of any competitor, so the pivots are fixed
+    # A = P^T L U with |L_ij| < 0.45/n and diag(U) in [1, 2]: at every step the
+    # unit-diagonal row beats every competitor by a factor > n (and L, U are
+    # well conditioned, so rounding in forming A cannot change that)
     rng = np.random.default_rng(3)
     for n in (700, 2100):
-        L = np.tril(rng.uniform(-0.45, 0.45, (n, n)), -1) + np.eye(n)
-        U = np.triu(rng.uniform(-1, 1, (n, n))) + np.diag(rng.uniform(1, 2, n))
+        L = np.tril(rng.uniform(-0.45, 0.45, (n, n)), -1) / n + np.eye(n)
+        U = np.triu(rng.uniform(-1, 1, (n, n)), 1) / n + np.diag(rng.uniform(1, 2, n))
         perm = rng.permutation(n)
         a = np.asfortranarray((L @ U)[np.argsort(perm)])
         lu = a.copy(order="F")
