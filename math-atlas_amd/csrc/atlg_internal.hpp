@@ -63,7 +63,8 @@ enum Slot : int {
   kSlotStageB = 5,
   kSlotStageC = 6,
   kSlotIpiv = 7,
-  kNumSlots = 8
+  kSlotFactInv = 8,   // diagonal-block inverses written by factorization leaves
+  kNumSlots = 9
 };
 void* scratch(hipStream_t s, Slot slot, size_t bytes);
 
@@ -84,15 +85,24 @@ void scale(hipStream_t s, long M, long N, double beta, double* C, long ldc,
 // C(i,j) <- C(j,i) for i > j (fills the lower triangle from the upper).
 void syreflect_upper(hipStream_t s, long N, double* C, long ldc);
 
+// TRSM on the matrix cores.  Diagonal blocks of order kTrsmNB are inverted
+// in one launch unless `dinv` supplies them: block b (rows/cols
+// [b*kTrsmNB, (b+1)*kTrsmNB) of op(A)) at dinv + b*kTrsmNB^2, column-major
+// with leading dimension kTrsmNB, holding inv(op(A)_bb) or, when
+// dinv_trans != 0, its transpose.
+constexpr int kTrsmNB = 128;
+// Writes inv(op(A)_bb) for every kTrsmNB block of the order-na triangle in
+// the layout trsm() takes as `dinv` (one launch, one workgroup per block).
+void tri_inverse_blocks(hipStream_t s, long na, const double* A, long lda, int ta, int uplo,
+                        int diag, double* Dinv, const int* skip = nullptr);
 void trsm(hipStream_t s, int side, int uplo, int ta, int diag, long M, long N,
           double alpha, const double* A, long lda, double* B, long ldb,
-          const int* skip = nullptr);
+          const int* skip = nullptr, const double* dinv = nullptr,
+          int dinv_trans = 0);
 
 void laswp(hipStream_t s, long N, double* A, long lda, int K1, int K2,
            const int* ipiv, int inci);
 
-// Adds `off` to ipiv[k1..k2) (device).
-void ipiv_add(hipStream_t s, int* ipiv, int k1, int k2, int off);
 
 void getrf(hipStream_t s, long M, long N, double* A, long lda, int* ipiv,
            int* dinfo);

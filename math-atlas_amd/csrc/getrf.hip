@@ -4,42 +4,56 @@
 // routines ATL_dgetf2 (:5204-5390), LU2 (:1301-1426) and the MN==1 case
 // (:1581-1600), ATL_dlaswp (:1053-1166) and ATL_dgetrs (:1631-1720).
 //
-// The recursion is ATLAS's: split the columns in half, factor the left half,
-// swap+TRSM+GEMM the right half, factor it, and back-apply its interchanges
-// to the left half.  The recursion stops at panels of at most PW columns,
-// which are factored by
-//   * panel_regs: one workgroup, one panel row per thread held in registers,
-//     pivot search as a wavefront argmax over cross-lane shuffles + LDS,
-//     used when the panel has at most 512 rows;
-//   * panel_step: one launch per column for taller panels; each launch
-//     applies the previous column's scaling and rank-1 update to its rows,
-//     finds its rows' candidate pivot, and the last workgroup to finish
-//     reduces the candidates, swaps the two panel rows and publishes the
-//     pivot.
+// The recursion is ATLAS's: split the columns, factor the left part, TRSM +
+// GEMM the right part, factor it.  It stops at panels of at most PW columns
+// ("leaves").  Two things differ from a line-by-line reading of getrfC:
+//
+//  * Row interchanges are applied by each leaf, at once, to every column of
+//    the matrix outside the leaf, instead of by the laswp calls of every
+//    recursion level.  This gives the same matrix: the columns to the right
+//    have not been touched yet when a leaf runs, the columns to the left are
+//    final L columns, and the swaps still happen in pivot order.  A leaf
+//    composes its swaps into one "row i receives row j" list, so a leaf's
+//    interchanges cost one gather/scatter pass over the other columns.
+//
+//  * A leaf panel is factored by one cooperative launch (panel_coop): the
+//    panel rows live in the registers of G <= 64 workgroups; per column each
+//    workgroup finds its candidate pivot (wavefront argmax over cross-lane
+//    shuffles, then LDS), publishes it with the candidate row, and after one
+//    grid barrier every workgroup picks the same global pivot, swaps, scales
+//    and applies the rank-1 update to its rows.  The barrier's wait is
+//    bounded; on overrun every workgroup leaves and *dinfo = INT_MIN reports
+//    the failure.
+//
 // Pivot choice follows cblas_idamax (AtlasBase/Antoine/atlrefblas.base:753+):
-// first index of the largest |a|, NaN never chosen.
+// first index of the largest |a|, NaN never chosen; the column is scaled by
+// the reciprocal pivot when |pivot| >= SAFMIN, else divided.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
 #include <cfloat>
 #include <climits>
+#include <vector>
 
 #include "atlg_internal.hpp"
 
 namespace atlg {
 namespace {
 
-constexpr int PW = 32;           // leaf panel width
-constexpr int SMALL_ROWS = 512;  // panel_regs limit (one row per thread)
+constexpr int PW = 32;            // leaf panel width
+constexpr int CP_THREADS = 256;   // workgroup of the cooperative panel
+constexpr int CP_MAXG = 64;       // workgroups of the cooperative panel
 constexpr int STEP_THREADS = 256;
-constexpr int KC = 256;          // interchanges composed per laswp chunk
-constexpr int CW = 16;           // columns per laswp workgroup
+constexpr int KC = 256;           // interchanges composed per laswp chunk
+constexpr int CW = 16;            // columns per laswp workgroup
+constexpr long kSpinLimit = 1L << 20;  // barrier polls (~0.1 s) before giving up
 
 // ---------------------------------------------------------------- helpers
 
 struct Cand {
-  double key;  // |a|, or -1 for NaN / out of range
+  double key;  // |a|, -1 for NaN, -2 for "no row"
   int idx;
+  int pad;
 };
 
 __device__ inline double pkey(double v) { return (v != v) ? -1.0 : fabs(v); }
@@ -71,7 +85,7 @@ __device__ inline void block_argmax(double& k, int& i, double* sk, int* si) {
   }
   __syncthreads();
   if (w == 0) {
-    double kk = lane < nw ? sk[lane] : -2.0;
+    double kk = lane < nw ? sk[lane] : -3.0;
     int ii = lane < nw ? si[lane] : INT_MAX;
     wave_argmax(kk, ii);
     if (lane == 0) {
@@ -85,186 +99,52 @@ __device__ inline void block_argmax(double& k, int& i, double* sk, int* si) {
   __syncthreads();
 }
 
-// ---------------------------------------------------------------- panels
-
-// Factors an M x w panel (M <= SMALL_ROWS, w <= PW) held one row per thread.
-__global__ void __launch_bounds__(SMALL_ROWS)
-panel_regs(int M, int w, double* __restrict__ A, long lda, int* __restrict__ ipiv,
-           int* __restrict__ dinfo, int info_off) {
-  __shared__ double prow[PW], jrow[PW];
-  __shared__ double sk[16];
-  __shared__ int si[16];
-  const int r = threadIdx.x;
-  double v[PW];
+// element j of a register row (j runtime, indices compile-time)
+__device__ __forceinline__ double rsel(const double (&r)[PW], int j) {
+  double x = 0.0;
 #pragma unroll
-  for (int c = 0; c < PW; c++) v[c] = (r < M && c < w) ? A[r + (size_t)c * lda] : 0.0;
-
-  const int mn = min(M, w);
-  for (int j = 0; j < mn; j++) {
-    double key = -1.0;
-#pragma unroll
-    for (int c = 0; c < PW; c++)
-      if (c == j) key = (r >= j && r < M) ? pkey(v[c]) : -2.0;
-    int idx = (r >= j && r < M) ? r : INT_MAX;
-    block_argmax(key, idx, sk, si);
-    const int p = idx;
-    if (r == p) {
-#pragma unroll
-      for (int c = 0; c < PW; c++) prow[c] = v[c];
-    }
-    if (r == j) {
-#pragma unroll
-      for (int c = 0; c < PW; c++) jrow[c] = v[c];
-    }
-    __syncthreads();
-    double piv = 0.0;
-#pragma unroll
-    for (int c = 0; c < PW; c++)
-      if (c == j) piv = prow[c];
-    if (r == 0) {
-      ipiv[j] = p;
-      if (piv == 0.0 && *dinfo == 0) *dinfo = info_off + j + 1;
-    }
-    if (piv != 0.0) {
-      if (r == j && p != j) {
-#pragma unroll
-        for (int c = 0; c < PW; c++) v[c] = prow[c];
-      } else if (r == p && p != j) {
-#pragma unroll
-        for (int c = 0; c < PW; c++) v[c] = jrow[c];
-      }
-      if (r > j && r < M) {
-        const bool recip = fabs(piv) >= DBL_MIN;
-        const double rinv = 1.0 / piv;
-        double l = 0.0;
-#pragma unroll
-        for (int c = 0; c < PW; c++)
-          if (c == j) {
-            v[c] = recip ? v[c] * rinv : v[c] / piv;
-            l = v[c];
-          }
-#pragma unroll
-        for (int c = 0; c < PW; c++)
-          if (c > j && c < w) v[c] -= l * prow[c];
-      }
-    }
-    __syncthreads();
-  }
-  if (r < M) {
-#pragma unroll
-    for (int c = 0; c < PW; c++)
-      if (c < w) A[r + (size_t)c * lda] = v[c];
-  }
+  for (int c = 0; c < PW; c++)
+    if (c == j) x = r[c];
+  return x;
 }
 
-// Pivot record published by the last workgroup of a column step.
-struct PivRec {
-  double piv;
-  int p;
-  int pad;
-};
-
-// One column step of a tall panel.  Launch j (0 <= j <= mn) first applies
-// column j-1's scaling and rank-1 update to rows [j, M), then (j < mn) finds
-// the pivot of column j.  `scr` holds per-workgroup candidates, the ticket
-// counter and the pivot record.
-__global__ void __launch_bounds__(STEP_THREADS)
-panel_step(int M, int w, int j, int mn, double* __restrict__ A, long lda,
-           int* __restrict__ ipiv, int* __restrict__ dinfo, int info_off,
-           Cand* __restrict__ cand, unsigned* __restrict__ ticket, PivRec* __restrict__ rec) {
-  __shared__ double urow[PW];
-  __shared__ double sk[4];
-  __shared__ int si[4];
-  __shared__ bool last;
-  const int t = threadIdx.x;
-  const int r = j + blockIdx.x * STEP_THREADS + t;  // this thread's row
-  const bool mine = r < M;
-
-  if (j > 0) {
-    const PivRec pr = rec[j - 1];
-    if (t < w) urow[t] = A[(j - 1) + (size_t)t * lda];
-    __syncthreads();
-    if (mine && pr.piv != 0.0) {
-      double* a = A + r;
-      double l = a[(size_t)(j - 1) * lda];
-      l = (fabs(pr.piv) >= DBL_MIN) ? l * (1.0 / pr.piv) : l / pr.piv;
-      a[(size_t)(j - 1) * lda] = l;
-      for (int c = j; c < w; c++) a[(size_t)c * lda] -= l * urow[c];
-    }
-  }
-  if (j >= mn) return;
-
-  double key = mine ? pkey(A[r + (size_t)j * lda]) : -2.0;
-  int idx = mine ? r : INT_MAX;
-  block_argmax(key, idx, sk, si);
-  if (t == 0) {
-    cand[blockIdx.x].key = key;
-    cand[blockIdx.x].idx = idx;
-    __threadfence();
-    const unsigned tk = atomicAdd(ticket, 1u);
-    last = (tk == gridDim.x - 1);
-  }
-  __syncthreads();
-  if (!last) return;
-
-  // last workgroup: reduce the candidates, swap rows j and p, publish
-  __threadfence();
-  double k2 = -2.0;
-  int i2 = INT_MAX;
-  for (int b = t; b < (int)gridDim.x; b += STEP_THREADS) {
-    const volatile Cand* cv = cand + b;
-    const double kb = cv->key;
-    const int ib = cv->idx;
-    if (better(kb, ib, k2, i2)) {
-      k2 = kb;
-      i2 = ib;
-    }
-  }
-  block_argmax(k2, i2, sk, si);
-  const int p = i2;
-  const double piv = A[p + (size_t)j * lda];
-  if (piv != 0.0 && p != j && t < w) {
-    double* a = A + (size_t)t * lda;
-    const double x = a[j];
-    a[j] = a[p];
-    a[p] = x;
-  }
-  if (t == 0) {
-    ipiv[j] = p;
-    if (piv == 0.0 && *dinfo == 0) *dinfo = info_off + j + 1;
-    rec[j].piv = piv;
-    rec[j].p = p;
-    *ticket = 0u;
-  }
+// coherent (agent-scope) loads/stores for data shared between workgroups
+__device__ inline double ld_agent(const double* p) {
+  const unsigned long long u = __hip_atomic_load(
+      reinterpret_cast<unsigned long long*>(const_cast<double*>(p)), __ATOMIC_RELAXED,
+      __HIP_MEMORY_SCOPE_AGENT);
+  return __longlong_as_double(static_cast<long long>(u));
+}
+__device__ inline void st_agent(double* p, double x) {
+  __hip_atomic_store(reinterpret_cast<unsigned long long*>(p),
+                     static_cast<unsigned long long>(__double_as_longlong(x)), __ATOMIC_RELAXED,
+                     __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ inline int ld_agent(const int* p) {
+  return __hip_atomic_load(const_cast<int*>(p), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ inline void st_agent(int* p, int x) {
+  __hip_atomic_store(p, x, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
-// ---------------------------------------------------------------- laswp
+// ------------------------------------------------------ swap composition
 
-// Composes interchanges ipiv[i] for i in [k1, k2) (ascending if inci > 0,
-// descending otherwise; at most KC of them) into the list of rows whose
-// content changes: after the swaps, row dst[t] holds the original row
-// src[t].  One workgroup; each thread owns one step.
-//   A_s: content at a_s = i_s after step s;  B_s: content at b_s = ipiv[i_s].
-//   A_s = content before step s at b_s,  B_s = content before s at a_s,
-// where "content before step s at x" is the value left by the latest earlier
-// step touching x, or x itself.  References are resolved by pointer jumping.
-__global__ void __launch_bounds__(KC)
-laswp_compose(const int* __restrict__ ipiv, int k1, int k2, int inci, int* __restrict__ out) {
-  __shared__ int pa[KC], pb[KC];   // positions of step s
-  __shared__ int val[2 * KC];      // >= 0: original row; < 0: -(ref+1)
-  __shared__ int cnt;
+// Composes k <= blockDim.x interchanges, step s swapping positions pa[s] and
+// pb[s] (in order s = 0, 1, ...), into the list of rows whose content
+// changes: afterwards row out[1+2t] holds the original row out[2+2t];
+// out[0] = count.  Every thread of the block must call it.
+//   A_s: content at pa[s] after step s;  B_s: content at pb[s] after step s.
+//   A_s = content before step s at pb[s], B_s = content before s at pa[s],
+// where "content before step s at x" is what the latest earlier step
+// touching x left there, or x itself.  References are resolved by pointer
+// jumping; a position's final content is what its last step left.
+__device__ void compose_swaps(const int* pa, const int* pb, int k, int* val, int* cnt,
+                              int* __restrict__ out) {
   const int s = threadIdx.x;
-  const int k = k2 - k1;
-  if (s == 0) cnt = 0;
-  if (s < k) {
-    const int i = inci > 0 ? k1 + s : k2 - 1 - s;
-    pa[s] = i;
-    pb[s] = ipiv[i];
-  }
-  __syncthreads();
+  if (s == 0) *cnt = 0;
   if (s < k) {
     const int a = pa[s], b = pb[s];
-    int ra = -1, rb = -1;  // latest earlier refs for b (-> A_s) and a (-> B_s)
+    int ra = -1, rb = -1;
     for (int q = s - 1; q >= 0 && (ra < 0 || rb < 0); q--) {
       if (ra < 0) {
         if (pa[q] == b) ra = 2 * q;
@@ -279,7 +159,7 @@ laswp_compose(const int* __restrict__ ipiv, int k1, int k2, int inci, int* __res
     val[2 * s + 1] = rb >= 0 ? -(rb + 1) : a;
   }
   __syncthreads();
-  for (int round = 0; round < 10; round++) {  // 2^10 > 2*KC
+  for (int span = 1; span < 2 * k; span *= 2) {  // pointer jumping
     int v0 = 0, v1 = 0;
     if (s < k) {
       v0 = val[2 * s];
@@ -302,22 +182,37 @@ laswp_compose(const int* __restrict__ ipiv, int k1, int k2, int inci, int* __res
       if (pb[q] == b || pa[q] == b) lb = false;
     }
     if (la && val[2 * s] != a) {
-      const int t = atomicAdd(&cnt, 1);
+      const int t = atomicAdd(cnt, 1);
       out[1 + 2 * t] = a;
       out[2 + 2 * t] = val[2 * s];
     }
     if (lb && val[2 * s + 1] != b) {
-      const int t = atomicAdd(&cnt, 1);
+      const int t = atomicAdd(cnt, 1);
       out[1 + 2 * t] = b;
       out[2 + 2 * t] = val[2 * s + 1];
     }
   }
   __syncthreads();
-  if (s == 0) out[0] = cnt;
+  if (s == 0) out[0] = *cnt;
 }
 
-// Applies a composed list to CW columns per workgroup: gather the source rows
-// into LDS, then scatter them to their destinations.
+// Interchanges ipiv[i], i in [k1, k2) (ascending if inci > 0, else
+// descending; at most KC of them) composed into a row list.
+__global__ void __launch_bounds__(KC)
+laswp_compose(const int* __restrict__ ipiv, int k1, int k2, int inci, int* __restrict__ out) {
+  __shared__ int pa[KC], pb[KC], val[2 * KC], cnt;
+  const int s = threadIdx.x, k = k2 - k1;
+  if (s < k) {
+    const int i = inci > 0 ? k1 + s : k2 - 1 - s;
+    pa[s] = i;
+    pb[s] = ipiv[i];
+  }
+  __syncthreads();
+  compose_swaps(pa, pb, k, val, &cnt, out);
+}
+
+// Applies a composed list (at most 2*KC rows) to CW columns per workgroup:
+// gather the source rows into LDS, then scatter them to their destinations.
 __global__ void __launch_bounds__(256)
 laswp_apply(long N, double* __restrict__ A, long lda, const int* __restrict__ lst) {
   __shared__ double buf[2 * KC * CW];
@@ -335,77 +230,401 @@ laswp_apply(long N, double* __restrict__ A, long lda, const int* __restrict__ ls
   }
 }
 
-__global__ void ipiv_add_kernel(int* ipiv, int k1, int k2, int off) {
-  const int i = k1 + blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < k2) ipiv[i] += off;
+// ---------------------------------------------------------------- panels
+
+// Global scratch of one cooperative panel launch.
+struct CoopBuf {
+  unsigned bar;  // barrier arrivals (reset to 0 before each launch)
+  int err;       // 1 when a barrier wait ran past its bound
+  int pad[2];
+  Cand cand[2][CP_MAXG];           // per column parity: workgroup candidates
+  double crow[2][CP_MAXG][PW];     // ... and their candidate rows
+  double jrow[2][PW];              // ... and row j
+};
+
+__global__ void coop_reset(CoopBuf* cb, int* list) {
+  cb->bar = 0u;
+  cb->err = 0;
+  list[0] = 0;
+}
+
+// Grid barrier of a cooperative launch: returns once `target` arrivals are
+// counted.  All data exchanged between workgroups goes through agent-scope
+// (coherence-point) atomic stores and loads, so the barrier itself needs no
+// cache write-back or invalidation: writers wait for their stores to
+// complete (workgroup-scope release), thread 0 counts the arrival and polls
+// the counter with relaxed loads.  The verdict is decided by thread 0 and
+// broadcast through LDS so that every thread of the workgroup takes the same
+// branch.  Returns false when the wait ran past its bound (in any
+// workgroup).
+__device__ bool grid_sync(CoopBuf* cb, unsigned target, int* verdict) {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    __hip_atomic_fetch_add(&cb->bar, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    long spins = 0;
+    while (__hip_atomic_load(&cb->bar, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < target) {
+      if (ld_agent(&cb->err)) break;
+      if (++spins > kSpinLimit) {
+        st_agent(&cb->err, 1);
+        break;
+      }
+      __builtin_amdgcn_s_sleep(1);
+    }
+    *verdict = (ld_agent(&cb->err) == 0);
+  }
+  __syncthreads();
+  return *verdict != 0;
+}
+
+// Factors the M x w (w <= PW) panel P (whose row 0 is absolute row `off` of
+// the matrix, on the diagonal).  Row i of the panel is held by global thread
+// i mod (G*256), RPT rows per thread.  Writes ipiv[off + j] (absolute),
+// *dinfo (first zero pivot), and the composed interchange list.
+template <int RPT>
+__global__ void __launch_bounds__(CP_THREADS)
+panel_coop(int M, int w, int off, double* __restrict__ P, long lda, int* __restrict__ ipiv,
+           int* __restrict__ dinfo, CoopBuf* __restrict__ cb, int* __restrict__ list) {
+  __shared__ double prow[PW], jrow[PW];
+  __shared__ double crows[CP_MAXG * PW];
+  __shared__ double sk[CP_THREADS / 64];
+  __shared__ int si[CP_THREADS / 64];
+  __shared__ int winner, verdict;
+  __shared__ int pa[PW], pb[PW], cval[2 * PW], ccnt;
+  const int t = threadIdx.x, G = gridDim.x, GT = G * CP_THREADS;
+  const int g = blockIdx.x * CP_THREADS + t;
+  double v[RPT][PW];
+#pragma unroll
+  for (int q = 0; q < RPT; q++) {
+    const int i = g + q * GT;
+#pragma unroll
+    for (int c = 0; c < PW; c++) v[q][c] = (i < M && c < w) ? P[i + (size_t)c * lda] : 0.0;
+  }
+  const int mn = min(M, w);
+  bool ok = true;
+  for (int j = 0; j < mn; j++) {
+    const int par = j & 1;
+    double key = -2.0;
+    int idx = INT_MAX;
+#pragma unroll
+    for (int q = 0; q < RPT; q++) {
+      const int i = g + q * GT;
+      if (i >= j && i < M) {
+        const double kq = pkey(rsel(v[q], j));
+        if (better(kq, i, key, idx)) {
+          key = kq;
+          idx = i;
+        }
+      }
+    }
+    block_argmax(key, idx, sk, si);
+    bool wrote = false;
+#pragma unroll
+    for (int q = 0; q < RPT; q++) {
+      const int i = g + q * GT;
+      if (i == idx) {
+#pragma unroll
+        for (int c = 0; c < PW; c++)
+          if (c < w) st_agent(&cb->crow[par][blockIdx.x][c], v[q][c]);
+        wrote = true;
+      }
+      if (i == j) {
+#pragma unroll
+        for (int c = 0; c < PW; c++)
+          if (c < w) st_agent(&cb->jrow[par][c], v[q][c]);
+        wrote = true;
+      }
+    }
+    if (t == 0) {
+      st_agent(&cb->cand[par][blockIdx.x].key, key);
+      st_agent(&cb->cand[par][blockIdx.x].idx, idx);
+      wrote = true;
+    }
+    (void)wrote;
+    if (!grid_sync(cb, (unsigned)G * (j + 1), &verdict)) {
+      ok = false;
+      break;
+    }
+    // one round trip: every workgroup reads all G candidates, all G
+    // candidate rows and row j, then picks the winner locally
+    for (int e = t; e < G * PW; e += CP_THREADS) {
+      const int b = e / PW, c = e - b * PW;
+      crows[e] = (c < w) ? ld_agent(&cb->crow[par][b][c]) : 0.0;
+    }
+    if (t < PW) jrow[t] = (t < w) ? ld_agent(&cb->jrow[par][t]) : 0.0;
+    if (t < 64) {
+      double kk = -3.0;
+      int ii = INT_MAX;
+      if (t < G) {
+        kk = ld_agent(&cb->cand[par][t].key);
+        ii = ld_agent(&cb->cand[par][t].idx);
+      }
+      wave_argmax(kk, ii);
+      if (t == 0) winner = ii;
+    }
+    __syncthreads();
+    // rows j..M-1 always hold a candidate, so p is in [j, M); the clamp only
+    // keeps a corrupted exchange from indexing outside the panel
+    const int p = (winner >= j && winner < M) ? winner : j;
+    const int bw = (p % GT) / CP_THREADS;  // workgroup holding row p
+    if (t < PW) prow[t] = crows[bw * PW + t];
+    __syncthreads();
+    const double piv = prow[j];
+    if (blockIdx.x == 0 && t == 0) {
+      ipiv[off + j] = off + p;
+      pa[j] = off + j;
+      pb[j] = off + p;
+      if (piv == 0.0 && *dinfo == 0) *dinfo = off + j + 1;
+    }
+    if (piv != 0.0) {
+      const bool recip = fabs(piv) >= DBL_MIN;
+      const double rinv = 1.0 / piv;
+#pragma unroll
+      for (int q = 0; q < RPT; q++) {
+        const int i = g + q * GT;
+        if (p != j) {
+          if (i == j) {
+#pragma unroll
+            for (int c = 0; c < PW; c++) v[q][c] = prow[c];
+          } else if (i == p) {
+#pragma unroll
+            for (int c = 0; c < PW; c++) v[q][c] = jrow[c];
+          }
+        }
+        if (i > j && i < M) {
+          double l = 0.0;
+#pragma unroll
+          for (int c = 0; c < PW; c++)  // scale column j (fused select keeps v in VGPRs)
+            if (c == j) {
+              v[q][c] = recip ? v[q][c] * rinv : v[q][c] / piv;
+              l = v[q][c];
+            }
+#pragma unroll
+          for (int c = 0; c < PW; c++)
+            if (c > j && c < w) v[q][c] -= l * prow[c];
+        }
+      }
+    }
+  }
+#pragma unroll
+  for (int q = 0; q < RPT; q++) {
+    const int i = g + q * GT;
+    if (i < M) {
+#pragma unroll
+      for (int c = 0; c < PW; c++)
+        if (c < w) P[i + (size_t)c * lda] = v[q][c];
+    }
+  }
+  if (!ok) {
+    if (t == 0) *dinfo = INT_MIN;
+    return;
+  }
+  if (blockIdx.x == 0) {
+    __syncthreads();
+    compose_swaps(pa, pb, mn, cval, &ccnt, list);
+  }
+}
+
+// Pivot record published by the last workgroup of a column step.
+struct PivRec {
+  double piv;
+  int p;
+  int pad;
+};
+
+// Fallback for panels taller than the cooperative kernel holds: one launch
+// per column.  Launch j (0 <= j <= mn) first applies column j-1's scaling
+// and rank-1 update to rows [j, M), then (j < mn) finds the pivot of column
+// j; the last workgroup to finish reduces the candidates, swaps rows j and
+// p across the panel and publishes the pivot (ipiv absolute: off + p).
+__global__ void __launch_bounds__(STEP_THREADS)
+panel_step(int M, int w, int j, int mn, int off, double* __restrict__ A, long lda,
+           int* __restrict__ ipiv, int* __restrict__ dinfo, Cand* __restrict__ cand,
+           unsigned* __restrict__ ticket, PivRec* __restrict__ rec) {
+  __shared__ double urow[PW];
+  __shared__ double sk[4];
+  __shared__ int si[4];
+  __shared__ bool last;
+  const int t = threadIdx.x;
+  const int r = j + blockIdx.x * STEP_THREADS + t;
+  const bool mine = r < M;
+  if (j > 0) {
+    const PivRec pr = rec[j - 1];
+    if (t < w) urow[t] = A[(j - 1) + (size_t)t * lda];
+    __syncthreads();
+    if (mine && pr.piv != 0.0) {
+      double* a = A + r;
+      double l = a[(size_t)(j - 1) * lda];
+      l = (fabs(pr.piv) >= DBL_MIN) ? l * (1.0 / pr.piv) : l / pr.piv;
+      a[(size_t)(j - 1) * lda] = l;
+      for (int c = j; c < w; c++) a[(size_t)c * lda] -= l * urow[c];
+    }
+  }
+  if (j >= mn) return;
+  double key = mine ? pkey(A[r + (size_t)j * lda]) : -2.0;
+  int idx = mine ? r : INT_MAX;
+  block_argmax(key, idx, sk, si);
+  if (t == 0) {
+    cand[blockIdx.x].key = key;
+    cand[blockIdx.x].idx = idx;
+    __threadfence();
+    const unsigned tk = atomicAdd(ticket, 1u);
+    last = (tk == gridDim.x - 1);
+  }
+  __syncthreads();
+  if (!last) return;
+  __threadfence();
+  double k2 = -2.0;
+  int i2 = INT_MAX;
+  for (int b = t; b < (int)gridDim.x; b += STEP_THREADS) {
+    const volatile Cand* cv = cand + b;
+    const double kb = cv->key;
+    const int ib = cv->idx;
+    if (better(kb, ib, k2, i2)) {
+      k2 = kb;
+      i2 = ib;
+    }
+  }
+  block_argmax(k2, i2, sk, si);
+  const int p = i2;
+  const double piv = A[p + (size_t)j * lda];
+  if (piv != 0.0 && p != j && t < w) {
+    double* a = A + (size_t)t * lda;
+    const double x = a[j];
+    a[j] = a[p];
+    a[p] = x;
+  }
+  if (t == 0) {
+    ipiv[off + j] = off + p;
+    if (piv == 0.0 && *dinfo == 0) *dinfo = off + j + 1;
+    rec[j].piv = piv;
+    rec[j].p = p;
+    *ticket = 0u;
+  }
 }
 
 __global__ void set_int_kernel(int* p, int v) { *p = v; }
 
 // ---------------------------------------------------------------- drivers
 
-struct PanelScratch {
-  Cand* cand;
+struct LU {
+  hipStream_t s;
+  double* A;
+  long lda, M, N;
+  int* ipiv;
+  int* dinfo;
+  CoopBuf* cb;
+  int* list;
+  Cand* cand;        // panel_step scratch
   unsigned* ticket;
   PivRec* rec;
+  double* Winv;      // inverses of the 128-blocks of L
+  std::vector<char> inv_done;
 };
 
-void panel(hipStream_t s, long M, int w, double* A, long lda, int* ipiv, int* dinfo,
-           int info_off, const PanelScratch& ps) {
-  const int mn = (int)std::min<long>(M, w);
-  if (M <= SMALL_ROWS) {
-    hipLaunchKernelGGL(panel_regs, dim3(1), dim3(SMALL_ROWS), 0, s, (int)M, w, A, lda, ipiv,
-                       dinfo, info_off);
+double* at(const LU& f, long i, long j) { return f.A + i + j * f.lda; }
+
+// Applies a composed list to all columns except [c0, c0 + w).
+void apply_outside(LU& f, long c0, long w, const int* lst) {
+  if (c0 > 0) {
+    hipLaunchKernelGGL(laswp_apply, dim3((unsigned)((c0 + CW - 1) / CW)), dim3(256), 0, f.s, c0,
+                       f.A, f.lda, lst);
     ATLG_LAUNCHED();
-    return;
   }
-  for (int j = 0; j <= mn; j++) {
-    const long rows = M - j;
-    const unsigned nb = (unsigned)std::max<long>(1, (rows + STEP_THREADS - 1) / STEP_THREADS);
-    hipLaunchKernelGGL(panel_step, dim3(nb), dim3(STEP_THREADS), 0, s, (int)M, w, j, mn, A, lda,
-                       ipiv, dinfo, info_off, ps.cand, ps.ticket, ps.rec);
+  const long r0 = c0 + w, nr = f.N - r0;
+  if (nr > 0) {
+    hipLaunchKernelGGL(laswp_apply, dim3((unsigned)((nr + CW - 1) / CW)), dim3(256), 0, f.s, nr,
+                       at(f, 0, r0), f.lda, lst);
     ATLG_LAUNCHED();
   }
 }
 
-void getrf_rec(hipStream_t s, long M, long N, double* A, long lda, int* ipiv, int* dinfo,
-               int info_off, const PanelScratch& ps) {
-  const long MN = std::min(M, N);
-  if (MN == 0) return;
-  if (N <= PW) {
-    panel(s, M, (int)N, A, lda, ipiv, dinfo, info_off, ps);
+template <int RPT>
+void launch_coop(LU& f, int M, int w, int off, int G) {
+  double* P = at(f, off, off);
+  long lda = f.lda;
+  void* args[] = {&M, &w, &off, &P, &lda, &f.ipiv, &f.dinfo, &f.cb, &f.list};
+  ATLG_HIP(hipLaunchCooperativeKernel(reinterpret_cast<const void*>(&panel_coop<RPT>), dim3(G),
+                                      dim3(CP_THREADS), args, 0, f.s));
+}
+
+// Factors the leaf panel at (off, off): m rows, w columns, and applies its
+// interchanges to every other column.
+void leaf(LU& f, long off, long m, int w) {
+  const long cap = (long)CP_MAXG * CP_THREADS;
+  if (m <= 4 * cap) {
+    const int rpt = m <= cap ? 1 : (m <= 2 * cap ? 2 : 4);
+    const int G = (int)((m + (long)CP_THREADS * rpt - 1) / ((long)CP_THREADS * rpt));
+    hipLaunchKernelGGL(coop_reset, dim3(1), dim3(1), 0, f.s, f.cb, f.list);
+    ATLG_LAUNCHED();
+    if (rpt == 1) launch_coop<1>(f, (int)m, w, (int)off, G);
+    else if (rpt == 2) launch_coop<2>(f, (int)m, w, (int)off, G);
+    else launch_coop<4>(f, (int)m, w, (int)off, G);
+    apply_outside(f, off, w, f.list);
     return;
   }
-  if (MN <= PW) {  // short and wide: factor the square part, fix up the rest
-    panel(s, M, (int)MN, A, lda, ipiv, dinfo, info_off, ps);
-    laswp(s, N - MN, A + MN * lda, lda, 0, (int)MN, ipiv, 1);
-    trsm(s, kLeft, kLower, kNoTrans, kUnit, MN, N - MN, 1.0, A, lda, A + MN * lda, lda);
+  const int mn = (int)std::min<long>(m, w);
+  double* P = at(f, off, off);
+  for (int j = 0; j <= mn; j++) {
+    const unsigned nb = (unsigned)std::max<long>(1, (m - j + STEP_THREADS - 1) / STEP_THREADS);
+    hipLaunchKernelGGL(panel_step, dim3(nb), dim3(STEP_THREADS), 0, f.s, (int)m, w, j, mn,
+                       (int)off, P, f.lda, f.ipiv, f.dinfo, f.cand, f.ticket, f.rec);
+    ATLG_LAUNCHED();
+  }
+  hipLaunchKernelGGL(laswp_compose, dim3(1), dim3(KC), 0, f.s, f.ipiv, (int)off, (int)off + mn,
+                     1, f.list);
+  ATLG_LAUNCHED();
+  apply_outside(f, off, w, f.list);
+}
+
+// U12 <- L11^-1 A12 for the node at `off` with nl pivot columns.
+void node_trsm(LU& f, long off, long nl, long nr) {
+  if (nl >= kTrsmNB && off % kTrsmNB == 0 && nl % kTrsmNB == 0) {
+    const long b0 = off / kTrsmNB, b1 = (off + nl) / kTrsmNB;
+    long first = b0;
+    while (first < b1 && f.inv_done[first]) first++;
+    if (first < b1) {
+      tri_inverse_blocks(f.s, (b1 - first) * kTrsmNB, at(f, first * kTrsmNB, first * kTrsmNB),
+                         f.lda, kNoTrans, kLower, kUnit,
+                         f.Winv + (size_t)first * kTrsmNB * kTrsmNB);
+      for (long b = first; b < b1; b++) f.inv_done[b] = 1;
+    }
+    trsm(f.s, kLeft, kLower, kNoTrans, kUnit, nl, nr, 1.0, at(f, off, off), f.lda,
+         at(f, off, off + nl), f.lda, nullptr, f.Winv + (size_t)b0 * kTrsmNB * kTrsmNB, 0);
+  } else {
+    trsm(f.s, kLeft, kLower, kNoTrans, kUnit, nl, nr, 1.0, at(f, off, off), f.lda,
+         at(f, off, off + nl), f.lda);
+  }
+}
+
+// Factors rows [off, M) x columns [off, off + n).
+void getrf_rec(LU& f, long off, long n) {
+  const long m = f.M - off, mn = std::min(m, n);
+  if (mn <= 0) return;
+  if (n <= PW) {
+    leaf(f, off, m, (int)n);
     return;
   }
-  long nl = ((MN / 2) / PW) * PW;
-  if (nl < PW) nl = PW;
-  const long nr = N - nl;
-  getrf_rec(s, M, nl, A, lda, ipiv, dinfo, info_off, ps);
-  double* Ac = A + nl * lda;
-  laswp(s, nr, Ac, lda, 0, (int)nl, ipiv, 1);
-  trsm(s, kLeft, kLower, kNoTrans, kUnit, nl, nr, 1.0, A, lda, Ac, lda);
-  gemm(s, kNoTrans, kNoTrans, M - nl, nr, nl, -1.0, A + nl, lda, Ac, lda, 1.0, Ac + nl, lda);
-  getrf_rec(s, M - nl, nr, Ac + nl, lda, ipiv + nl, dinfo, info_off + (int)nl, ps);
-  ipiv_add(s, ipiv, (int)nl, (int)MN, (int)nl);
-  laswp(s, nl, A, lda, (int)nl, (int)MN, ipiv, 1);
+  if (mn <= PW) {  // short and wide: factor the square part, solve the rest
+    leaf(f, off, m, (int)mn);
+    node_trsm(f, off, mn, n - mn);
+    return;
+  }
+  long nl;
+  if (mn >= 2 * kTrsmNB) nl = ((mn / 2) / kTrsmNB) * kTrsmNB;
+  else nl = std::max<long>(PW, ((mn / 2) / PW) * PW);
+  const long nr = n - nl;
+  getrf_rec(f, off, nl);
+  node_trsm(f, off, nl, nr);
+  gemm(f.s, kNoTrans, kNoTrans, m - nl, nr, nl, -1.0, at(f, off + nl, off), f.lda,
+       at(f, off, off + nl), f.lda, 1.0, at(f, off + nl, off + nl), f.lda);
+  getrf_rec(f, off + nl, nr);
 }
 
 }  // namespace
 
 void set_int(hipStream_t s, int* p, int v) {
   hipLaunchKernelGGL(set_int_kernel, dim3(1), dim3(1), 0, s, p, v);
-  ATLG_LAUNCHED();
-}
-
-void ipiv_add(hipStream_t s, int* ipiv, int k1, int k2, int off) {
-  if (k2 <= k1) return;
-  const int n = k2 - k1;
-  hipLaunchKernelGGL(ipiv_add_kernel, dim3((n + 255) / 256), dim3(256), 0, s, ipiv, k1, k2, off);
   ATLG_LAUNCHED();
 }
 
@@ -434,15 +653,31 @@ void laswp(hipStream_t s, long N, double* A, long lda, int K1, int K2, const int
 void getrf(hipStream_t s, long M, long N, double* A, long lda, int* ipiv, int* dinfo) {
   set_int(s, dinfo, 0);
   if (M == 0 || N == 0) return;
+  LU f;
+  f.s = s;
+  f.A = A;
+  f.lda = lda;
+  f.M = M;
+  f.N = N;
+  f.ipiv = ipiv;
+  f.dinfo = dinfo;
   const long maxblk = (M + STEP_THREADS - 1) / STEP_THREADS + 1;
-  char* base = static_cast<char*>(
-      scratch(s, kSlotPanel, 256 + sizeof(Cand) * maxblk + sizeof(PivRec) * PW));
-  PanelScratch ps;
-  ps.ticket = reinterpret_cast<unsigned*>(base);
-  ps.rec = reinterpret_cast<PivRec*>(base + 64);
-  ps.cand = reinterpret_cast<Cand*>(base + 64 + sizeof(PivRec) * PW + 64);
-  set_int(s, reinterpret_cast<int*>(ps.ticket), 0);
-  getrf_rec(s, M, N, A, lda, ipiv, dinfo, 0, ps);
+  const size_t b_cb = (sizeof(CoopBuf) + 255) & ~size_t(255);
+  const size_t b_list = 256 + sizeof(int) * (1 + 4 * KC);
+  const size_t b_rec = 256 + sizeof(PivRec) * PW;
+  const size_t b_cand = sizeof(Cand) * maxblk;
+  char* base = static_cast<char*>(scratch(s, kSlotPanel, b_cb + b_list + 256 + b_rec + b_cand));
+  f.cb = reinterpret_cast<CoopBuf*>(base);
+  f.list = reinterpret_cast<int*>(base + b_cb);
+  f.ticket = reinterpret_cast<unsigned*>(base + b_cb + b_list);
+  f.rec = reinterpret_cast<PivRec*>(base + b_cb + b_list + 256);
+  f.cand = reinterpret_cast<Cand*>(base + b_cb + b_list + 256 + b_rec);
+  set_int(s, reinterpret_cast<int*>(f.ticket), 0);
+  const long nblk = (std::min(M, N) + kTrsmNB - 1) / kTrsmNB;
+  f.Winv = static_cast<double*>(
+      scratch(s, kSlotFactInv, sizeof(double) * std::max<long>(nblk, 1) * kTrsmNB * kTrsmNB));
+  f.inv_done.assign(nblk + 1, 0);
+  getrf_rec(f, 0, N);
 }
 
 void getrs(hipStream_t s, int trans, long N, long NRHS, const double* A, long lda,

@@ -7,7 +7,8 @@
 // diagonals and runs everything off the diagonal as amm GEMM; this version
 // goes one step further:
 //   1. one launch inverts every NB x NB diagonal block of op(A) in LDS
-//      (one workgroup per block), writing full NB x NB inverses;
+//      (one workgroup per block: 8x8 register inverses, then recursive
+//      doubling), writing full NB x NB inverses;
 //   2. a recursive halving driver (the same split as trsm_recL/R) applies
 //      each inverse to its block row/column as an in-place GEMM and updates
 //      the rest with large GEMMs.
@@ -17,13 +18,14 @@
 #include <algorithm>
 
 #include "atlg_internal.hpp"
+#include "tri_inv.hpp"
 
 namespace atlg {
 namespace {
 
-constexpr int NB = 128;        // diagonal block order (<= GEMM tile edge)
-constexpr int LDT = NB + 1;    // LDS row pitch
-constexpr int TRI_THREADS = 256;
+constexpr int NB = kTrsmNB;    // diagonal block order (<= GEMM tile edge)
+constexpr int TRI_THREADS = kTriThreads;
+static_assert(NB == kTriN, "TRSM blocks are inverted by the 128-block inverter");
 
 // Element (i, j) of op(A).
 __device__ inline double opa(const double* A, long lda, bool trans, long i, long j) {
@@ -39,51 +41,34 @@ __global__ void __launch_bounds__(TRI_THREADS)
 trtri_diag_kernel(int na, const double* __restrict__ A, long lda, int trans, int lower,
                   int unit, double* __restrict__ Dinv, const int* __restrict__ skip) {
   if (skip && *skip) return;
-  __shared__ double T[NB * LDT];
-  __shared__ double xs[NB];
+  __shared__ double S[kTriN * kTriLD];
   const int b = blockIdx.x, t = threadIdx.x;
   const long o = (long)b * NB;
   const int n = (int)min((long)NB, na - o);
   const bool tr = trans != 0;
-
-  // T[i][j] (stored T[i*LDT + j]) = lower-triangular form of the block.
-  for (int idx = t; idx < n * n; idx += TRI_THREADS) {
-    const int i = idx % n, j = idx / n;
+  for (int idx = t; idx < NB * NB; idx += TRI_THREADS) {
+    const int i = idx % NB, j = idx / NB;
     double v = 0.0;
     if (i >= j) {
-      if (lower)
-        v = (i == j && unit) ? 1.0 : opa(A, lda, tr, o + i, o + j);
-      else
-        v = (i == j && unit) ? 1.0 : opa(A, lda, tr, o + n - 1 - i, o + n - 1 - j);
+      if (i < n && j < n) {
+        if (i == j && unit) v = 1.0;
+        else v = lower ? opa(A, lda, tr, o + i, o + j)
+                       : opa(A, lda, tr, o + n - 1 - i, o + n - 1 - j);
+      } else if (i == j) {
+        v = 1.0;
+      }
     }
-    T[i * LDT + j] = v;
+    S[i * kTriLD + j] = v;
   }
   __syncthreads();
-
-  // In-place inversion, last column first: for column j, with the trailing
-  // block already inverted, x <- -d * inv(T22) * T(j+1:n, j), d = 1/T(j,j).
-  for (int j = n - 1; j >= 0; j--) {
-    const double d = unit ? 1.0 : 1.0 / T[j * LDT + j];
-    for (int i = j + 1 + t; i < n; i += TRI_THREADS) xs[i] = T[i * LDT + j];
-    __syncthreads();
-    for (int i = j + 1 + t; i < n; i += TRI_THREADS) {
-      double y = 0.0;
-      for (int k = j + 1; k <= i; k++) y += T[i * LDT + k] * xs[k];
-      T[i * LDT + j] = -d * y;
-    }
-    if (t == 0) T[j * LDT + j] = d;
-    __syncthreads();
-  }
-
+  lower_inverse_inplace(S);
   double* D = Dinv + (size_t)b * NB * NB;
   for (int idx = t; idx < NB * NB; idx += TRI_THREADS) {
     const int i = idx % NB, j = idx / NB;
     double v = 0.0;
     if (i < n && j < n) {
-      if (lower)
-        v = (i >= j) ? T[i * LDT + j] : 0.0;
-      else
-        v = (i <= j) ? T[(n - 1 - i) * LDT + (n - 1 - j)] : 0.0;
+      if (lower) v = (i >= j) ? S[i * kTriLD + j] : 0.0;
+      else v = (i <= j) ? S[(n - 1 - i) * kTriLD + (n - 1 - j)] : 0.0;
     }
     D[i + (size_t)j * NB] = v;
   }
@@ -98,6 +83,7 @@ struct Ctx {
   long ldb;
   long M, N;   // B is M x N
   const double* Dinv;
+  int dtrans;  // kTrans when Dinv holds the transposes of inv(op(A)_bb)
   const int* skip;
 };
 
@@ -123,11 +109,11 @@ void solve(const Ctx& c, long r0, long n, double alpha) {
     const double* D = c.Dinv + (size_t)(r0 / NB) * NB * NB;
     if (c.left) {
       double* Bb = c.B + r0;
-      gemm(c.s, kNoTrans, kNoTrans, n, c.N, n, alpha, D, NB, Bb, c.ldb, 0.0, Bb, c.ldb, 0,
+      gemm(c.s, c.dtrans, kNoTrans, n, c.N, n, alpha, D, NB, Bb, c.ldb, 0.0, Bb, c.ldb, 0,
            c.skip);
     } else {
       double* Bb = c.B + r0 * c.ldb;
-      gemm(c.s, kNoTrans, kNoTrans, c.M, n, n, alpha, Bb, c.ldb, D, NB, 0.0, Bb, c.ldb, 0,
+      gemm(c.s, kNoTrans, c.dtrans, c.M, n, n, alpha, Bb, c.ldb, D, NB, 0.0, Bb, c.ldb, 0,
            c.skip);
     }
     return;
@@ -176,8 +162,20 @@ void solve(const Ctx& c, long r0, long n, double alpha) {
 
 }  // namespace
 
+void tri_inverse_blocks(hipStream_t s, long na, const double* A, long lda, int ta, int uplo,
+                        int diag, double* Dinv, const int* skip) {
+  if (na <= 0) return;
+  const bool tr = is_trans(ta);
+  const long nblk = (na + NB - 1) / NB;
+  hipLaunchKernelGGL(trtri_diag_kernel, dim3((unsigned)nblk), dim3(TRI_THREADS), 0, s, (int)na,
+                     A, lda, tr ? 1 : 0, (uplo == kLower) != tr ? 1 : 0, diag == kUnit ? 1 : 0,
+                     Dinv, skip);
+  ATLG_LAUNCHED();
+}
+
 void trsm(hipStream_t s, int side, int uplo, int ta, int diag, long M, long N, double alpha,
-          const double* A, long lda, double* B, long ldb, const int* skip) {
+          const double* A, long lda, double* B, long ldb, const int* skip, const double* dinv,
+          int dinv_trans) {
   if (M <= 0 || N <= 0) return;
   if (alpha == 0.0) {
     scale(s, M, N, 0.0, B, ldb, 0, skip);
@@ -196,13 +194,15 @@ void trsm(hipStream_t s, int side, int uplo, int ta, int diag, long M, long N, d
   c.N = N;
   c.skip = skip;
   const long na = c.left ? M : N;
-  const long nblk = (na + NB - 1) / NB;
-  double* Dinv = static_cast<double*>(scratch(s, kSlotTrsmInv, sizeof(double) * nblk * NB * NB));
-  hipLaunchKernelGGL(trtri_diag_kernel, dim3((unsigned)nblk), dim3(TRI_THREADS), 0, s, (int)na,
-                     A, lda, c.trans ? 1 : 0, (uplo == kLower) != c.trans ? 1 : 0,
-                     diag == kUnit ? 1 : 0, Dinv, skip);
-  ATLG_LAUNCHED();
-  c.Dinv = Dinv;
+  c.dtrans = dinv ? (dinv_trans ? kTrans : kNoTrans) : kNoTrans;
+  if (dinv) {
+    c.Dinv = dinv;
+  } else {
+    const long nblk = (na + NB - 1) / NB;
+    double* D = static_cast<double*>(scratch(s, kSlotTrsmInv, sizeof(double) * nblk * NB * NB));
+    tri_inverse_blocks(s, na, A, lda, ta, uplo, diag, D, skip);
+    c.Dinv = D;
+  }
   solve(c, 0, na, alpha);
 }
 
