@@ -8,8 +8,10 @@ are resident in HBM before timing starts.
 
 Also reported in the same line (`sweep`): DGEMM N=4096/8192, tall-skinny
 M=65536 N=K=256, DTRSM L/L/N/U N=16384 with 16384 RHS, DGETRF and DPOTRF
-N=16384 (TFLOP/s with ATLAS's flop formulas), and the HBM rate of the laswp
-row interchanges inside DGETRF.
+N=16384 (TFLOP/s with ATLAS's flop formulas); the batched uneven SPD solves
+(10000 systems, n ~ U{16..512}, 4 RHS, FP64 and FP32 + FP64 refinement); and
+the differentiable Cholesky log-likelihood at N=8192 (forward, and forward +
+backward with the factor stored or recomputed).
 
 `roofline` is computed from HIP events recorded on the stream the DGEMM
 kernels run on; `cpu_baseline` times the CPU oracle's textbook DGEMM on a
@@ -41,6 +43,7 @@ def parse():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-cols", type=int, default=160)
     ap.add_argument("--only", default="", help="profile helper: run only this sweep item")
+    ap.add_argument("--ir-n", type=int, default=32768, help="order of the mixed-precision solve")
     ap.add_argument("--copy-calib", action="store_true",
                     help="profile helper: three 1 GiB device-to-device copies (HBM counter scale)")
     return ap.parse_args()
@@ -70,7 +73,7 @@ def timed(torch, fn, reps, warm=1):
     return best
 
 
-def sweep(torch, G, only=""):
+def sweep(torch, G, only="", ir_n=32768):
     out = {}
 
     def want(k):
@@ -141,7 +144,206 @@ def sweep(torch, G, only=""):
         out["dpotrf_16384_ms"] = ms
         out["dpotrf_16384_info"] = int(info.item())
         del s, s0
+    if want("ir"):
+        out.update(bench_ir(torch, G, ir_n))
+    if want("batched"):
+        out.update(bench_batched(torch, G))
+    if want("cholgrad"):
+        out.update(bench_cholgrad(torch))
     torch.cuda.empty_cache()
+    return out
+
+
+def haar(torch, n, g):
+    """Haar-random orthogonal n x n (QR of a Gaussian, R's diagonal signs
+    folded into Q)."""
+    q, r = torch.linalg.qr(torch.randn((n, n), generator=g, dtype=torch.float64, device="cuda"))
+    q.mul_(torch.sign(torch.diagonal(r)).unsqueeze(0))
+    del r
+    return q
+
+
+def bench_ir(torch, G, n):
+    """Mixed-precision dgesv at order n, one RHS b iid normal(0,1),
+    A = U diag(s) V^T with U, V Haar orthogonal and s_i = kappa^(-i/(n-1)),
+    kappa in {1e2, 1e6, 1e10}: plain FP64 dgesv (getrf + getrs) against the
+    FP32-LU and BF16-LU refinement (atlg_dsgesv), with each solution's
+    normwise backward error ||b - A x||_inf / (||A||_inf ||x||_inf + ||b||_inf)."""
+    import time as _t
+    g = torch.Generator(device="cuda")
+    g.manual_seed(32768)
+    t0 = _t.time()
+    U = haar(torch, n, g)
+    V = haar(torch, n, g)
+    out = {"ir_n": n, "ir_setup_s": 0.0}
+    b = torch.randn(n, generator=g, dtype=torch.float64, device="cuda")
+    ipiv = torch.zeros(n, dtype=torch.int32, device="cuda")
+    info = torch.zeros(1, dtype=torch.int32, device="cuda")
+    x = torch.zeros(n, dtype=torch.float64, device="cuda")
+    ar = torch.arange(n, dtype=torch.float64, device="cuda")
+    for kappa in (1e2, 1e6, 1e10):
+        sv = torch.pow(torch.tensor(kappa, dtype=torch.float64, device="cuda"), -ar / (n - 1))
+        # column-major A = row-major A^T = V diag(s) U^T
+        Acm = torch.mm(V * sv.unsqueeze(0), U.t().contiguous())
+        A = Acm.clone()
+        anrm = float(Acm.abs().sum(dim=0).max())  # ||A||_inf: column sums of A^T
+        tag = "%.0e" % kappa
+
+        def berr():
+            r = b - torch.mv(Acm.t(), x)  # A x with A^T stored row-major
+            return float(r.abs().max() / (anrm * x.abs().max() + b.abs().max()))
+
+        def run64():
+            A.copy_(Acm)
+            x.copy_(b)
+            G.dgetrf(n, n, A, n, ipiv, info)
+            G.dgetrs(G.NoTrans, n, 1, A, n, ipiv, x, n)
+
+        its = {}
+
+        def runmix(lp):
+            A.copy_(Acm)
+            its[lp] = G.dsgesv(n, 1, A, n, ipiv, b, n, x, n, info, 30, lp)
+
+        ms_copy = timed(torch, lambda: A.copy_(Acm), 2)
+        if kappa == 1e2:
+            out["ir_setup_s"] = round(_t.time() - t0, 1)
+            ms64 = timed(torch, run64, 1) - ms_copy
+            out["ir_dgesv_fp64_ms"] = ms64
+            out["ir_dgesv_fp64_berr"] = berr()
+            out["ir_dgetrf_fp64_tflops"] = (2.0 * n ** 3 / 3) / ms64 / 1e9
+        for lp, name in ((0, "fp32"), (1, "bf16")):
+            ms = timed(torch, lambda: runmix(lp), 1) - ms_copy
+            out["ir_k%s_%s_ms" % (tag, name)] = ms
+            out["ir_k%s_%s_iter" % (tag, name)] = its[lp]
+            out["ir_k%s_%s_berr" % (tag, name)] = berr()
+            out["ir_k%s_%s_speedup" % (tag, name)] = out["ir_dgesv_fp64_ms"] / ms
+        del Acm, A
+    del U, V
+    return out
+
+
+def batched_workload(torch, nsys=10000, lo=16, hi=512, nrhs=4, seed=4):
+    """nsys SPD systems, n ~ U{lo..hi} (seeded), A = B B^T / n + I with B iid
+    normal(0,1), packed column-major (lda = n) by order; nrhs normal
+    right-hand sides per system."""
+    import numpy as np
+    rng = np.random.default_rng(seed)
+    ns = rng.integers(lo, hi + 1, size=nsys)
+    order = np.argsort(ns, kind="stable")
+    offs = np.zeros(nsys, dtype=np.int64)
+    tot = 0
+    for k in order:
+        offs[k] = tot
+        tot += int(ns[k]) ** 2
+    A0 = torch.empty(tot, dtype=torch.float64, device="cuda")
+    g = torch.Generator(device="cuda")
+    g.manual_seed(seed)
+    for n in np.unique(ns):
+        idx = np.nonzero(ns == n)[0]
+        n = int(n)
+        o = int(offs[idx].min())
+        b = torch.randn((len(idx), n, n), generator=g, dtype=torch.float64, device="cuda")
+        a = torch.bmm(b, b.transpose(1, 2).contiguous()).div_(n)
+        a.diagonal(dim1=1, dim2=2).add_(1.0)
+        A0[o:o + len(idx) * n * n] = a.reshape(-1)
+    boffs = np.cumsum(np.concatenate([[0], ns[:-1] * nrhs])).astype(np.int64)
+    B0 = torch.randn(int(ns.sum()) * nrhs, generator=g, dtype=torch.float64, device="cuda")
+    return [int(x) for x in ns], offs, A0, boffs, B0
+
+
+def bench_batched(torch, G):
+    """Batched uneven SPD solves: 10000 systems, n ~ U{16..512}, 4 RHS,
+    potrf + potrs in one persistent launch (FP64), and FP32 factor + FP64
+    refinement (atlg_dsposv_batched)."""
+    import numpy as np
+    nrhs = 4
+    ns, offs, A0, boffs, B0 = batched_workload(torch, nrhs=nrhs)
+    A, B = A0.clone(), B0.clone()
+    X = torch.zeros_like(B0)
+    pa = [A.data_ptr() + 8 * int(o) for o in offs]
+    pb = [B.data_ptr() + 8 * int(o) for o in boffs]
+    px = [X.data_ptr() + 8 * int(o) for o in boffs]
+    nsys = len(ns)
+    info = torch.zeros(nsys, dtype=torch.int32, device="cuda")
+    it = torch.zeros(nsys, dtype=torch.int32, device="cuda")
+
+    def restore():
+        A.copy_(A0)
+        B.copy_(B0)
+
+    def run64():
+        restore()
+        G.dposv_batched(ns, pa, ns, pb, ns, nrhs, info)
+
+    def runmix():
+        restore()
+        G.dsposv_batched(ns, pa, ns, pb, ns, px, ns, nrhs, info, it)
+
+    ms_copy = timed(torch, restore, 3)
+    ms64 = timed(torch, run64, 3) - ms_copy
+    bad64 = int((info != 0).sum().item())
+    msmix = timed(torch, runmix, 3) - ms_copy
+    badmix = int((info != 0).sum().item())
+    n = np.asarray(ns, dtype=np.float64)
+    flops = float((n ** 3 / 3 + n ** 2 / 2 + n / 6).sum() + (2 * n ** 2 * nrhs).sum())
+    byts = float((8 * n ** 2 + 4 * n ** 2 + 16 * n * nrhs).sum())
+    its = it.cpu().numpy()
+    out = {
+        "batched_posv_10000_ms": ms64,
+        "batched_posv_10000_systems_per_s": nsys / ms64 * 1e3,
+        "batched_posv_10000_tflops": flops / ms64 / 1e9,
+        "batched_posv_10000_GBps": byts / ms64 / 1e6,
+        "batched_posv_10000_nonzero_info": bad64,
+        "batched_dsposv_10000_ms": msmix,
+        "batched_dsposv_10000_fallbacks": int((its < 0).sum()),
+        "batched_dsposv_10000_max_iter": int(its.max()) if len(its) else 0,
+        "batched_dsposv_10000_nonzero_info": badmix,
+    }
+    # backward error of the refined solutions on a few systems (FP64 check)
+    worst = 0.0
+    for k in range(0, nsys, 997):
+        m = ns[k]
+        a = A0[int(offs[k]):int(offs[k]) + m * m].view(m, m)
+        x = X[int(boffs[k]):int(boffs[k]) + m * nrhs].view(nrhs, m).t()
+        b = B0[int(boffs[k]):int(boffs[k]) + m * nrhs].view(nrhs, m).t()
+        r = (b - a @ x).abs().amax(dim=0)
+        den = a.abs().sum(dim=1).amax() * x.abs().amax(dim=0) + b.abs().amax(dim=0)
+        worst = max(worst, float((r / den).max()))
+    out["batched_dsposv_10000_sample_backward_error"] = worst
+    del A, B, X, A0, B0
+    return out
+
+
+def bench_cholgrad(torch, n=8192):
+    """Differentiable Cholesky log-likelihood: RBF kernel (length-scale 0.2)
+    over n points iid U[0,1]^3 plus 1e-2 I, y iid normal; forward and
+    forward + backward through the hand-written adjoint kernels."""
+    from atlas_gpu import autograd as AG
+    g = torch.Generator(device="cuda")
+    g.manual_seed(5)
+    x = torch.rand((n, 3), generator=g, dtype=torch.float64, device="cuda")
+    d2 = torch.cdist(x, x).pow_(2)
+    K0 = torch.exp(-d2 / (2 * 0.2 ** 2))
+    K0.diagonal().add_(1e-2)
+    del d2
+    y = torch.randn(n, generator=g, dtype=torch.float64, device="cuda")
+    out = {}
+    ms_f = timed(torch, lambda: AG.gaussian_loglik(K0, y, store_factor=True), 3)
+    out["cholgrad_8192_forward_ms"] = ms_f
+    out["cholgrad_8192_forward_potrf_tflops"] = (n ** 3 / 3.0) / ms_f / 1e9
+    for store in (True, False):
+        K = K0.clone().requires_grad_(True)
+        yy = y.clone().requires_grad_(True)
+
+        def fb():
+            K.grad = None
+            yy.grad = None
+            AG.gaussian_loglik(K, yy, store_factor=store).backward()
+        out["cholgrad_8192_fwd_bwd_ms_%s" % ("stored" if store else "recomputed")] = \
+            timed(torch, fb, 3)
+        del K, yy
+    del K0
     return out
 
 
@@ -174,7 +376,7 @@ def main():
     assert G.lib().atlg_init() == 0
 
     if args.only:
-        print(json.dumps(sweep(torch, G, args.only)), flush=True)
+        print(json.dumps(sweep(torch, G, args.only, args.ir_n)), flush=True)
         return
     if args.copy_calib:
         src = torch.ones(1 << 27, dtype=torch.float64, device="cuda")
@@ -260,7 +462,7 @@ def main():
     torch.cuda.empty_cache()
     if not args.no_sweep:
         out["sweep"] = {k: (round(v, 4) if isinstance(v, float) else v)
-                        for k, v in sweep(torch, G).items()}
+                        for k, v in sweep(torch, G, "", args.ir_n).items()}
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(n, args.cpu_cols)
     print(json.dumps(out), flush=True)

@@ -70,6 +70,103 @@ int atlg_dpotrf(int Uplo, long N, double *A, long lda, int *dinfo,
 int atlg_dpotrs(int Uplo, long N, long NRHS, const double *A, long lda,
                 double *B, long ldb, atlg_stream_t stream);
 
+/* ---- Adjoints of the Cholesky path (no ATLAS counterpart) -------------
+ * For K = L L^T with L lower (as atlg_dpotrf with Lower leaves it):
+ * atlg_dpotrf_adjoint: on entry G holds Lbar (lower triangle read, upper
+ *   ignored); on exit G holds the symmetric Kbar = sym(L^-T Phi(L^T Lbar)
+ *   L^-1), Phi = lower triangle with halved diagonal.
+ * atlg_dtrsm_adjoint: for X = L^-1 B (N x NRHS): Xbar is overwritten by
+ *   Bbar = L^-T Xbar, and -tril(Bbar X^T) is added to Lbar (if not NULL).
+ * atlg_dlogdet_chol: *out (device) = log det K = 2 sum log L_ii.
+ * atlg_dlogdet_chol_adjoint: adds 2 (*dbar) / L_ii to the diagonal of Lbar
+ *   (dbar in device memory). */
+int atlg_dpotrf_adjoint(long N, const double *L, long ldl, double *G, long ldg,
+                        atlg_stream_t stream);
+int atlg_dtrsm_adjoint(long N, long NRHS, const double *L, long ldl,
+                       const double *X, long ldx, double *Xbar, long ldxb,
+                       double *Lbar, long ldlb, atlg_stream_t stream);
+int atlg_dlogdet_chol(long N, const double *L, long ldl, double *out,
+                      atlg_stream_t stream);
+int atlg_dlogdet_chol_adjoint(long N, const double *L, long ldl,
+                              const double *dbar, double *Lbar, long ldlb,
+                              atlg_stream_t stream);
+
+/* ---- Single precision and the mixed-precision solver (the FP32 path is
+ * the ATL_s instance of the same ATLAS routines: ATL_sgemm, ATL_sgetrf,
+ * ATL_sgetrs; AtlasBase/Clint/atlas-lp.base:1497-1720) ------------------
+ * atlg_sgemm: as atlg_dgemm in FP32 (FP32 MFMA).
+ * atlg_sgemm_bf16: C = alpha*bf16(A)*bf16(B) + beta*C, NoTrans/NoTrans,
+ *   operands rounded to bfloat16 (nearest even), FP32 accumulation and C.
+ * atlg_sgetrf: as atlg_dgetrf in FP32; gemm_prec 0 = FP32 trailing updates,
+ *   1 = BF16 trailing updates with FP32 accumulation.
+ * atlg_sgetrs: as atlg_dgetrs in FP32.
+ * atlg_dsgesv: LAPACK dsgesv semantics.  Factors float(A) with atlg_sgetrf
+ *   (gemm_prec = lowprec), refines X with FP64 residuals B - A X until
+ *   ||r_j||_inf < ||x_j||_inf ||A||_inf eps sqrt(N) for every column j (at
+ *   most itermax corrections; itermax <= 0 means 30).  *iter (host) = the
+ *   corrections used, or -2 (A overflows FP32), -3 (FP32 factor singular),
+ *   -(itermax+1) (no convergence); when negative, A holds the FP64 LU
+ *   factors and X the FP64 dgetrs solution.  ipiv holds the pivots of
+ *   whichever factor was used.  Synchronizes the stream. */
+int atlg_sgemm(int TA, int TB, long M, long N, long K, float alpha,
+               const float *A, long lda, const float *B, long ldb, float beta,
+               float *C, long ldc, atlg_stream_t stream);
+int atlg_sgemm_bf16(long M, long N, long K, float alpha, const float *A,
+                    long lda, const float *B, long ldb, float beta, float *C,
+                    long ldc, atlg_stream_t stream);
+int atlg_sgetrf(long M, long N, float *A, long lda, int *ipiv, int *dinfo,
+                int gemm_prec, atlg_stream_t stream);
+int atlg_sgetrs(int Trans, long N, long NRHS, const float *A, long lda,
+                const int *ipiv, float *B, long ldb, atlg_stream_t stream);
+int atlg_dsgesv(long N, long NRHS, double *A, long lda, int *ipiv,
+                const double *B, long ldb, double *X, long ldx, int *dinfo,
+                int itermax, int lowprec, int *iter, atlg_stream_t stream);
+
+/* ---- Batched SPD solvers (no ATLAS counterpart; per system the same
+ * lower Cholesky as ATL_dpotrf and the two solves of ATL_dpotrs,
+ * AtlasBase/Clint/atlas-lp.base:332-360, 969-1050) ----------------------
+ * nsys independent systems of uneven order in ONE persistent launch.
+ * n, lda, ldb, ldx: host arrays of nsys ints; A, B, X: host arrays of nsys
+ * device pointers (column-major, lower triangle of A read and written);
+ * info, iter: device arrays of nsys ints.  Systems are taken longest first
+ * from a device work queue.  Per system, info = 0 or the order of the first
+ * leading minor that is not positive definite (that system is not solved).
+ * Returns -(argument position) for an illegal argument (n[k] < 0,
+ * lda[k] < max(1,n[k]), ...), or ATLG_EHIP. */
+int atlg_dpotrf_batched(int nsys, const int *n, double *const *A,
+                        const int *lda, int *info, atlg_stream_t stream);
+int atlg_dpotrs_batched(int nsys, const int *n, double *const *A,
+                        const int *lda, double *const *B, const int *ldb,
+                        int nrhs, atlg_stream_t stream);
+int atlg_dposv_batched(int nsys, const int *n, double *const *A,
+                       const int *lda, double *const *B, const int *ldb,
+                       int nrhs, int *info, atlg_stream_t stream);
+int atlg_spotrf_batched(int nsys, const int *n, float *const *A,
+                        const int *lda, int *info, atlg_stream_t stream);
+int atlg_sposv_batched(int nsys, const int *n, float *const *A,
+                       const int *lda, float *const *B, const int *ldb,
+                       int nrhs, int *info, atlg_stream_t stream);
+/* Mixed precision (LAPACK dsposv semantics per system): the lower triangle
+ * of the FP64 A is factored in FP32, X is refined with FP64 residuals
+ * b - A x until ||r||_inf < ||x||_inf ||A||_inf eps sqrt(n) for every
+ * right-hand side (at most itermax corrections; itermax <= 0 means 30).
+ * iter[k] = corrections used (>= 0; A and B untouched), or -2 (A overflows
+ * FP32), -3 (FP32 factor failed), -(itermax+1) (no convergence): those
+ * systems are factored and solved in FP64, A then holding the FP64 factor.
+ * Synchronizes the stream before returning. */
+int atlg_dsposv_batched(int nsys, const int *n, double *const *A,
+                        const int *lda, const double *const *B,
+                        const int *ldb, double *const *X, const int *ldx,
+                        int nrhs, int itermax, int *info, int *iter,
+                        atlg_stream_t stream);
+/* Decision table of the batched solvers (prec 0 = FP64, 1 = FP32): systems
+ * with n <= t16_max use 16-wide block columns, wider ones 32; wg_per_cu
+ * persistent workgroups per compute unit.  The defaults come from the
+ * empirical search's generated header; set overrides them for this
+ * process (the search itself times candidates through it). */
+int atlg_batched_tune_get(int prec, int *t16_max, int *wg_per_cu);
+int atlg_batched_tune_set(int prec, int t16_max, int wg_per_cu);
+
 #ifdef __cplusplus
 }
 #endif

@@ -70,6 +70,25 @@ _SIGS = {
     "atlg_dgetrs": (_i, [_i, _l, _l, _v, _l, _v, _v, _l, _v]),
     "atlg_dpotrf": (_i, [_i, _l, _v, _l, _v, _v]),
     "atlg_dpotrs": (_i, [_i, _l, _l, _v, _l, _v, _l, _v]),
+    "atlg_dpotrf_adjoint": (_i, [_l, _v, _l, _v, _l, _v]),
+    "atlg_dtrsm_adjoint": (_i, [_l, _l, _v, _l, _v, _l, _v, _l, _v, _l, _v]),
+    "atlg_dlogdet_chol": (_i, [_l, _v, _l, _v, _v]),
+    "atlg_dlogdet_chol_adjoint": (_i, [_l, _v, _l, _v, _v, _l, _v]),
+    "atlg_sgemm": (_i, [_i, _i, _l, _l, _l, ctypes.c_float, _v, _l, _v, _l, ctypes.c_float, _v,
+                        _l, _v]),
+    "atlg_sgemm_bf16": (_i, [_l, _l, _l, ctypes.c_float, _v, _l, _v, _l, ctypes.c_float, _v, _l,
+                             _v]),
+    "atlg_sgetrf": (_i, [_l, _l, _v, _l, _v, _v, _i, _v]),
+    "atlg_sgetrs": (_i, [_i, _l, _l, _v, _l, _v, _v, _l, _v]),
+    "atlg_dsgesv": (_i, [_l, _l, _v, _l, _v, _v, _l, _v, _l, _v, _i, _i, _ip, _v]),
+    "atlg_dpotrf_batched": (_i, [_i, _ip, _v, _ip, _v, _v]),
+    "atlg_dpotrs_batched": (_i, [_i, _ip, _v, _ip, _v, _ip, _i, _v]),
+    "atlg_dposv_batched": (_i, [_i, _ip, _v, _ip, _v, _ip, _i, _v, _v]),
+    "atlg_spotrf_batched": (_i, [_i, _ip, _v, _ip, _v, _v]),
+    "atlg_sposv_batched": (_i, [_i, _ip, _v, _ip, _v, _ip, _i, _v, _v]),
+    "atlg_dsposv_batched": (_i, [_i, _ip, _v, _ip, _v, _ip, _v, _ip, _i, _i, _v, _v, _v]),
+    "atlg_batched_tune_get": (_i, [_i, _ip, _ip]),
+    "atlg_batched_tune_set": (_i, [_i, _i, _i]),
 }
 
 _LIB = None
@@ -277,3 +296,124 @@ def dpotrf(uplo, n, a, lda, info, stream=None):
 def dpotrs(uplo, n, nrhs, a, lda, b, ldb, stream=None):
     _check(lib().atlg_dpotrs(uplo, n, nrhs, _ptr(a), lda, _ptr(b), ldb, _stream(stream)),
            "atlg_dpotrs")
+
+
+def dpotrf_adjoint(n, L, ldl, G, ldg, stream=None):
+    _check(lib().atlg_dpotrf_adjoint(n, _ptr(L), ldl, _ptr(G), ldg, _stream(stream)),
+           "atlg_dpotrf_adjoint")
+
+
+def dtrsm_adjoint(n, nrhs, L, ldl, X, ldx, Xbar, ldxb, Lbar, ldlb, stream=None):
+    _check(lib().atlg_dtrsm_adjoint(n, nrhs, _ptr(L), ldl, _ptr(X), ldx, _ptr(Xbar), ldxb,
+                                    _ptr(Lbar), ldlb, _stream(stream)), "atlg_dtrsm_adjoint")
+
+
+def dlogdet_chol(n, L, ldl, out, stream=None):
+    _check(lib().atlg_dlogdet_chol(n, _ptr(L), ldl, _ptr(out), _stream(stream)),
+           "atlg_dlogdet_chol")
+
+
+def dlogdet_chol_adjoint(n, L, ldl, dbar, Lbar, ldlb, stream=None):
+    _check(lib().atlg_dlogdet_chol_adjoint(n, _ptr(L), ldl, _ptr(dbar), _ptr(Lbar), ldlb,
+                                           _stream(stream)), "atlg_dlogdet_chol_adjoint")
+
+
+# ---------------------------------------------------------------- batched SPD solvers
+# n, lda, ldb, ldx: sequences of ints (host); A, B, X: sequences of device
+# addresses (ints, e.g. tensor.data_ptr() + byte offset) or tensors; info and
+# iter: int32 device tensors of len(n).
+
+def _ints(seq):
+    a = np.ascontiguousarray(np.asarray(seq, dtype=np.int32))
+    return a, a.ctypes.data_as(_ip)
+
+
+def _ptrs(seq):
+    if seq is None:
+        return None, ctypes.c_void_p(0)
+    vals = [(x.data_ptr() if hasattr(x, "data_ptr") else int(x)) for x in seq]
+    a = (ctypes.c_void_p * max(1, len(vals)))(*vals)
+    return a, ctypes.cast(a, ctypes.c_void_p)
+
+
+def dpotrf_batched(n, A, lda, info, stream=None):
+    (kn, pn), (ka, pa), (kl, pl) = _ints(n), _ptrs(A), _ints(lda)
+    _check(lib().atlg_dpotrf_batched(len(kn), pn, pa, pl, _ptr(info), _stream(stream)),
+           "atlg_dpotrf_batched")
+
+
+def dpotrs_batched(n, A, lda, B, ldb, nrhs, stream=None):
+    (kn, pn), (ka, pa), (kl, pl), (kb, pb), (klb, plb) = (_ints(n), _ptrs(A), _ints(lda),
+                                                          _ptrs(B), _ints(ldb))
+    _check(lib().atlg_dpotrs_batched(len(kn), pn, pa, pl, pb, plb, nrhs, _stream(stream)),
+           "atlg_dpotrs_batched")
+
+
+def dposv_batched(n, A, lda, B, ldb, nrhs, info, stream=None):
+    (kn, pn), (ka, pa), (kl, pl), (kb, pb), (klb, plb) = (_ints(n), _ptrs(A), _ints(lda),
+                                                          _ptrs(B), _ints(ldb))
+    _check(lib().atlg_dposv_batched(len(kn), pn, pa, pl, pb, plb, nrhs, _ptr(info),
+                                    _stream(stream)), "atlg_dposv_batched")
+
+
+def spotrf_batched(n, A, lda, info, stream=None):
+    (kn, pn), (ka, pa), (kl, pl) = _ints(n), _ptrs(A), _ints(lda)
+    _check(lib().atlg_spotrf_batched(len(kn), pn, pa, pl, _ptr(info), _stream(stream)),
+           "atlg_spotrf_batched")
+
+
+def sposv_batched(n, A, lda, B, ldb, nrhs, info, stream=None):
+    (kn, pn), (ka, pa), (kl, pl), (kb, pb), (klb, plb) = (_ints(n), _ptrs(A), _ints(lda),
+                                                          _ptrs(B), _ints(ldb))
+    _check(lib().atlg_sposv_batched(len(kn), pn, pa, pl, pb, plb, nrhs, _ptr(info),
+                                    _stream(stream)), "atlg_sposv_batched")
+
+
+def dsposv_batched(n, A, lda, B, ldb, X, ldx, nrhs, info, iter, itermax=30, stream=None):
+    (kn, pn), (ka, pa), (kl, pl), (kb, pb), (klb, plb), (kx, px), (klx, plx) = (
+        _ints(n), _ptrs(A), _ints(lda), _ptrs(B), _ints(ldb), _ptrs(X), _ints(ldx))
+    _check(lib().atlg_dsposv_batched(len(kn), pn, pa, pl, pb, plb, px, plx, nrhs, itermax,
+                                     _ptr(info), _ptr(iter), _stream(stream)),
+           "atlg_dsposv_batched")
+
+
+def batched_tune(prec):
+    t, w = ctypes.c_int(0), ctypes.c_int(0)
+    _check(lib().atlg_batched_tune_get(prec, ctypes.byref(t), ctypes.byref(w)),
+           "atlg_batched_tune_get")
+    return t.value, w.value
+
+
+def set_batched_tune(prec, t16_max, wg_per_cu):
+    _check(lib().atlg_batched_tune_set(prec, t16_max, wg_per_cu), "atlg_batched_tune_set")
+
+
+# ---------------------------------------------------------------- single / mixed precision
+
+def sgemm(ta, tb, m, n, k, alpha, a, lda, b, ldb, beta, c, ldc, stream=None):
+    _check(lib().atlg_sgemm(ta, tb, m, n, k, alpha, _ptr(a), lda, _ptr(b), ldb, beta, _ptr(c),
+                            ldc, _stream(stream)), "atlg_sgemm")
+
+
+def sgemm_bf16(m, n, k, alpha, a, lda, b, ldb, beta, c, ldc, stream=None):
+    _check(lib().atlg_sgemm_bf16(m, n, k, alpha, _ptr(a), lda, _ptr(b), ldb, beta, _ptr(c), ldc,
+                                 _stream(stream)), "atlg_sgemm_bf16")
+
+
+def sgetrf(m, n, a, lda, ipiv, info, gemm_prec=0, stream=None):
+    _check(lib().atlg_sgetrf(m, n, _ptr(a), lda, _ptr(ipiv), _ptr(info), gemm_prec,
+                             _stream(stream)), "atlg_sgetrf")
+
+
+def sgetrs(trans, n, nrhs, a, lda, ipiv, b, ldb, stream=None):
+    _check(lib().atlg_sgetrs(trans, n, nrhs, _ptr(a), lda, _ptr(ipiv), _ptr(b), ldb,
+                             _stream(stream)), "atlg_sgetrs")
+
+
+def dsgesv(n, nrhs, a, lda, ipiv, b, ldb, x, ldx, info, itermax=30, lowprec=0, stream=None):
+    """Returns ITER (LAPACK dsgesv meaning)."""
+    it = ctypes.c_int(0)
+    _check(lib().atlg_dsgesv(n, nrhs, _ptr(a), lda, _ptr(ipiv), _ptr(b), ldb, _ptr(x), ldx,
+                             _ptr(info), itermax, lowprec, ctypes.byref(it), _stream(stream)),
+           "atlg_dsgesv")
+    return it.value

@@ -1,0 +1,126 @@
+// Reverse-mode (adjoint) kernels for the Cholesky path: potrf, trsm with a
+// Cholesky factor, and logdet.  They have no ATLAS counterpart; they extend
+// the level-3 routines so a Gaussian log-likelihood
+//   l(K, y) = -1/2 y^T K^-1 y - 1/2 log det K
+// can be differentiated on the device.  With K = L L^T (lower) and
+// a = L^-1 y, the pieces are
+//   logdet:  d = 2 sum log L_ii            -> Lbar_ii += 2 dbar / L_ii
+//   trsm:    a = L^-1 y                     -> ybar = L^-T abar,
+//                                              Lbar -= tril(ybar a^T)
+//   potrf:   L = chol(K)                    -> Kbar = sym(L^-T Phi(L^T Lbar) L^-1)
+// where Phi takes the lower triangle with the diagonal halved and
+// sym(X) = (X + X^T)/2 (blocked form of Murray, "Differentiation of the
+// Cholesky decomposition", 2016).  Every product runs on the MFMA GEMM and
+// TRSM of this library.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+
+#include "atlg_internal.hpp"
+
+namespace atlg {
+namespace {
+
+// dst = tril(src) (zero above the diagonal)
+__global__ void tril_copy_kernel(long N, const double* src, long lds,
+                                 double* dst, long ldd) {  // may run in place
+  const long total = N * N;
+  for (long e = blockIdx.x * (long)blockDim.x + threadIdx.x; e < total;
+       e += (long)gridDim.x * blockDim.x) {
+    const long i = e % N, j = e / N;
+    dst[i + j * ldd] = (i >= j) ? src[i + j * lds] : 0.0;
+  }
+}
+
+// X <- Phi(X): zero above the diagonal, halve the diagonal
+__global__ void phi_kernel(long N, double* __restrict__ X, long ldx) {
+  const long total = N * N;
+  for (long e = blockIdx.x * (long)blockDim.x + threadIdx.x; e < total;
+       e += (long)gridDim.x * blockDim.x) {
+    const long i = e % N, j = e / N;
+    double* x = X + i + j * ldx;
+    if (i < j) *x = 0.0;
+    else if (i == j) *x *= 0.5;
+  }
+}
+
+// dst = (X + X^T) / 2, full square
+__global__ void sym_kernel(long N, const double* __restrict__ X, long ldx,
+                           double* __restrict__ dst, long ldd) {
+  const long total = N * N;
+  for (long e = blockIdx.x * (long)blockDim.x + threadIdx.x; e < total;
+       e += (long)gridDim.x * blockDim.x) {
+    const long i = e % N, j = e / N;
+    dst[i + j * ldd] = 0.5 * (X[i + j * ldx] + X[j + i * ldx]);
+  }
+}
+
+// out = 2 sum log L_ii (one workgroup, fixed summation order)
+__global__ void __launch_bounds__(1024)
+logdet_kernel(long N, const double* __restrict__ L, long ldl, double* __restrict__ out) {
+  __shared__ double part[1024];
+  double s = 0.0;
+  for (long i = threadIdx.x; i < N; i += blockDim.x) s += log(L[i + i * ldl]);
+  part[threadIdx.x] = s;
+  __syncthreads();
+  for (int w = blockDim.x / 2; w > 0; w >>= 1) {
+    if ((int)threadIdx.x < w) part[threadIdx.x] += part[threadIdx.x + w];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) *out = 2.0 * part[0];
+}
+
+// Lbar_ii += 2 * dbar / L_ii
+__global__ void logdet_adjoint_kernel(long N, const double* __restrict__ L, long ldl,
+                                      const double* __restrict__ dbar, double* __restrict__ Lbar,
+                                      long ldlb) {
+  const long i = blockIdx.x * (long)blockDim.x + threadIdx.x;
+  if (i < N) Lbar[i + i * ldlb] += 2.0 * (*dbar) / L[i + i * ldl];
+}
+
+unsigned grid_for(long total) {
+  return (unsigned)std::min<long>((total + 255) / 256, 8192);
+}
+
+}  // namespace
+
+void potrf_adjoint(hipStream_t s, long N, const double* L, long ldl, double* G, long ldg) {
+  if (N <= 0) return;
+  double* W = static_cast<double*>(scratch(s, kSlotAdjoint, sizeof(double) * 2 * N * N));
+  double* Lc = W;          // tril(L)
+  double* P = W + N * N;   // Phi(L^T Lbar), then the adjoint
+  const unsigned g = grid_for(N * N);
+  hipLaunchKernelGGL(tril_copy_kernel, dim3(g), dim3(256), 0, s, N, L, ldl, Lc, N);
+  ATLG_LAUNCHED();
+  hipLaunchKernelGGL(tril_copy_kernel, dim3(g), dim3(256), 0, s, N, G, ldg, G, ldg);
+  ATLG_LAUNCHED();
+  gemm(s, kTrans, kNoTrans, N, N, N, 1.0, Lc, N, G, ldg, 0.0, P, N);
+  hipLaunchKernelGGL(phi_kernel, dim3(g), dim3(256), 0, s, N, P, N);
+  ATLG_LAUNCHED();
+  trsm(s, kLeft, kLower, kTrans, kNonUnit, N, N, 1.0, L, ldl, P, N);
+  trsm(s, kRight, kLower, kNoTrans, kNonUnit, N, N, 1.0, L, ldl, P, N);
+  hipLaunchKernelGGL(sym_kernel, dim3(g), dim3(256), 0, s, N, P, N, G, ldg);
+  ATLG_LAUNCHED();
+}
+
+void trsm_adjoint(hipStream_t s, long N, long NRHS, const double* L, long ldl, const double* X,
+                  long ldx, double* Xbar, long ldxb, double* Lbar, long ldlb) {
+  if (N <= 0 || NRHS <= 0) return;
+  trsm(s, kLeft, kLower, kTrans, kNonUnit, N, NRHS, 1.0, L, ldl, Xbar, ldxb);
+  if (Lbar) gemm(s, kNoTrans, kTrans, N, N, NRHS, -1.0, Xbar, ldxb, X, ldx, 1.0, Lbar, ldlb, 1);
+}
+
+void logdet_chol(hipStream_t s, long N, const double* L, long ldl, double* out) {
+  hipLaunchKernelGGL(logdet_kernel, dim3(1), dim3(1024), 0, s, N, L, ldl, out);
+  ATLG_LAUNCHED();
+}
+
+void logdet_chol_adjoint(hipStream_t s, long N, const double* L, long ldl, const double* dbar,
+                         double* Lbar, long ldlb) {
+  if (N <= 0) return;
+  hipLaunchKernelGGL(logdet_adjoint_kernel, dim3((unsigned)((N + 255) / 256)), dim3(256), 0, s,
+                     N, L, ldl, dbar, Lbar, ldlb);
+  ATLG_LAUNCHED();
+}
+
+}  // namespace atlg

@@ -64,23 +64,31 @@ enum Slot : int {
   kSlotStageC = 6,
   kSlotIpiv = 7,
   kSlotFactInv = 8,   // diagonal-block inverses written by factorization leaves
-  kNumSlots = 9
+  kSlotAdjoint = 9,   // work matrices of the Cholesky adjoint
+  kSlotBatchCtr = 10,   // work-queue counter of the batched solvers
+  kSlotBatchDesc = 11,  // per-system descriptors of the batched solvers
+  kSlotBatchWork = 12,  // FP32 factors and right-hand sides of the mixed solver
+  kSlotMixed = 13,      // FP32 factor, residual partials and norms of dsgesv
+  kSlotBf16 = 14,       // BF16 operand copies of the low-precision update
+  kNumSlots = 15
 };
 void* scratch(hipStream_t s, Slot slot, size_t bytes);
 
 // ---- kernels / drivers (device pointers, column-major) --------------------
 
-// GEMM family.  `mode`: 0 = full C, 1 = lower triangle only (SYRK),
-// 2 = upper triangle only.  `skip` (device int, may be null): when *skip != 0
-// the launch does nothing.  C may alias B when M <= 128 and beta == 0, or
-// alias A when N <= 128 and beta == 0 (used by TRSM's in-place diagonal
-// apply).
-void gemm(hipStream_t s, int TA, int TB, long M, long N, long K, double alpha,
-          const double* A, long lda, const double* B, long ldb, double beta,
-          double* C, long ldc, int mode = 0, const int* skip = nullptr);
+// GEMM family (S = double on the FP64 MFMA, S = float on the FP32 MFMA).
+// `mode`: 0 = full C, 1 = lower triangle only (SYRK), 2 = upper triangle
+// only.  `skip` (device int, may be null): when *skip != 0 the launch does
+// nothing.  C may alias B when M <= 128 and beta == 0, or alias A when
+// N <= 128 and beta == 0 (used by TRSM's in-place diagonal apply).
+template <class S>
+void gemm(hipStream_t s, int TA, int TB, long M, long N, long K, S alpha, const S* A, long lda,
+          const S* B, long ldb, S beta, S* C, long ldc, int mode = 0,
+          const int* skip = nullptr);
 // C <- beta*C on the M x N (or triangular per mode) part; beta==0 zeroes.
-void scale(hipStream_t s, long M, long N, double beta, double* C, long ldc,
-           int mode = 0, const int* skip = nullptr);
+template <class S>
+void scale(hipStream_t s, long M, long N, S beta, S* C, long ldc, int mode = 0,
+           const int* skip = nullptr);
 
 // C(i,j) <- C(j,i) for i > j (fills the lower triangle from the upper).
 void syreflect_upper(hipStream_t s, long N, double* C, long ldc);
@@ -93,24 +101,76 @@ void syreflect_upper(hipStream_t s, long N, double* C, long ldc);
 constexpr int kTrsmNB = 128;
 // Writes inv(op(A)_bb) for every kTrsmNB block of the order-na triangle in
 // the layout trsm() takes as `dinv` (one launch, one workgroup per block).
-void tri_inverse_blocks(hipStream_t s, long na, const double* A, long lda, int ta, int uplo,
-                        int diag, double* Dinv, const int* skip = nullptr);
-void trsm(hipStream_t s, int side, int uplo, int ta, int diag, long M, long N,
-          double alpha, const double* A, long lda, double* B, long ldb,
-          const int* skip = nullptr, const double* dinv = nullptr,
-          int dinv_trans = 0);
+template <class S>
+void tri_inverse_blocks(hipStream_t s, long na, const S* A, long lda, int ta, int uplo, int diag,
+                        S* Dinv, const int* skip = nullptr);
+template <class S>
+void trsm(hipStream_t s, int side, int uplo, int ta, int diag, long M, long N, S alpha,
+          const S* A, long lda, S* B, long ldb, const int* skip = nullptr,
+          const S* dinv = nullptr, int dinv_trans = 0);
 
-void laswp(hipStream_t s, long N, double* A, long lda, int K1, int K2,
-           const int* ipiv, int inci);
+template <class S>
+void laswp(hipStream_t s, long N, S* A, long lda, int K1, int K2, const int* ipiv, int inci);
 
-
-void getrf(hipStream_t s, long M, long N, double* A, long lda, int* ipiv,
-           int* dinfo);
-void getrs(hipStream_t s, int trans, long N, long NRHS, const double* A,
-           long lda, const int* ipiv, double* B, long ldb);
+// Precision of getrf's trailing-update GEMMs: the scalar type's own MFMA,
+// or (float only) BF16 operands with FP32 accumulation.
+enum GemmPrec : int { kGemmNative = 0, kGemmBf16 = 1 };
+template <class S>
+void getrf(hipStream_t s, long M, long N, S* A, long lda, int* ipiv, int* dinfo,
+           int gemm_prec = kGemmNative);
+template <class S>
+void getrs(hipStream_t s, int trans, long N, long NRHS, const S* A, long lda, const int* ipiv,
+           S* B, long ldb);
 void potrf(hipStream_t s, int uplo, long N, double* A, long lda, int* dinfo);
 void potrs(hipStream_t s, int uplo, long N, long NRHS, const double* A,
            long lda, double* B, long ldb);
+
+// Adjoints of the Cholesky path (lower factor L, K = L L^T):
+//   potrf_adjoint: G holds Lbar (lower triangle read) and receives
+//     Kbar = sym(L^-T Phi(L^T Lbar) L^-1) as a full symmetric matrix;
+//   trsm_adjoint: for X = L^-1 B, overwrites Xbar with Bbar = L^-T Xbar and
+//     adds -tril(Bbar X^T) to Lbar (when Lbar is not null);
+//   logdet_chol: *out = 2 sum log L_ii; its adjoint adds 2 dbar / L_ii to
+//     the diagonal of Lbar.
+void potrf_adjoint(hipStream_t s, long N, const double* L, long ldl, double* G, long ldg);
+void trsm_adjoint(hipStream_t s, long N, long NRHS, const double* L, long ldl, const double* X,
+                  long ldx, double* Xbar, long ldxb, double* Lbar, long ldlb);
+void logdet_chol(hipStream_t s, long N, const double* L, long ldl, double* out);
+void logdet_chol_adjoint(hipStream_t s, long N, const double* L, long ldl, const double* dbar,
+                         double* Lbar, long ldlb);
+
+// Batched SPD solvers (lower triangle; one persistent launch).  Host arrays
+// n/lda/ldb and host arrays of device pointers A/B; `info` is a device array
+// of nsys ints.  mode: 0 potrf, 1 potrf + potrs, 2 potrs with a given
+// factor.  Bsrc (may be null) is copied into B (same ldb) before the solve.
+// `orig` (may be null) maps list position -> index into info.
+struct BatchTune {
+  int t16_max;    // systems with n <= t16_max use 16-wide block columns, 32 otherwise
+  int wg_per_cu;  // persistent workgroups per compute unit
+};
+BatchTune batch_tune(bool single);
+void set_batch_tune(bool single, const BatchTune& t);
+template <class S>
+void posv_batched(hipStream_t s, int mode, int nsys, const int* n, S* const* A, const int* lda,
+                  S* const* B, const int* ldb, const S* const* Bsrc, int nrhs, int* info,
+                  const BatchTune& tune, const int* orig = nullptr);
+// FP32 factor + FP64 refinement (LAPACK dsposv semantics per system); iter is
+// a device array.  Synchronizes the stream (the fallback set is decided on
+// the host).
+void dsposv_batched(hipStream_t s, int nsys, const int* n, double* const* A, const int* lda,
+                    const double* const* B, const int* ldb, double* const* X, const int* ldx,
+                    int nrhs, int* info, int* iter, int itermax, const BatchTune& tune);
+
+// C(f32) = alpha * bf16(A) * bf16(B) + beta * C, A M x K and B K x N
+// column-major FP32 (NoTrans, NoTrans), FP32 accumulation.
+void gemm_bf16(hipStream_t s, long M, long N, long K, float alpha, const float* A, long lda,
+               const float* B, long ldb, float beta, float* C, long ldc);
+
+// Mixed-precision dgesv (LAPACK dsgesv semantics): FP32 LU (lowprec != 0:
+// BF16 trailing updates), FP64 residual refinement, FP64 fallback.  *iter_out
+// receives ITER.  Synchronizes the stream.
+int dsgesv(hipStream_t s, long N, long NRHS, double* A, long lda, int* ipiv, const double* B,
+           long ldb, double* X, long ldx, int* dinfo, int itermax, int lowprec, int* iter_out);
 
 // zero an int on device (kernel, graph-capture safe)
 void set_int(hipStream_t s, int* p, int v);

@@ -1,0 +1,657 @@
+// Batched Cholesky factor/solve of many independent SPD systems of uneven
+// order in one persistent launch.  ATLAS factors one matrix per call
+// (ATL_dpotrf / ATL_dpotrs, AtlasBase/Clint/atlas-lp.base:332-360, 969-1050);
+// the per-system arithmetic here is the same lower Cholesky A = L L^T and the
+// same pair of triangular solves, and the mixed-precision driver follows
+// LAPACK's dsposv refinement (single-precision factor, double-precision
+// residual, stop when ||r||_inf < ||x||_inf * ||A||_inf * eps * sqrt(n) for
+// every right-hand side, at most 30 iterations).
+//
+// Work distribution: the host orders the systems longest first (n^3 is the
+// work) and every workgroup pulls the next system from an atomic counter
+// until the queue is drained, so every workgroup exits after the last system.
+// One workgroup (kBW waves) owns a system from start to finish.
+//
+// Per system, left-looking by block columns of width T (16 or 32, chosen per
+// system from the tuned decision table):
+//   U: C = A[J:n, J:J+T] - L[J:n, 0:J] L[J:J+T, 0:J]^T   (MFMA 16x16x4 tiles,
+//      operands straight from global memory, lower part written back);
+//   F: one wave factors the T x T diagonal block in LDS and forms its inverse;
+//   T: L[J+T:n, J:J+T] = C * inv(L_JJ)^T                 (MFMA, inverse in LDS).
+// The solve sweeps block rows: one wave solves the diagonal block for all
+// right-hand sides with shuffles, then every thread updates the remaining
+// rows (forward: row-parallel, backward: column-parallel over L^T).
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cfloat>
+#include <cmath>
+#include <cstring>
+#include <mutex>
+#include <numeric>
+#include <vector>
+
+#include "atlg_internal.hpp"
+#include "atlg_batched_tune.h"
+
+namespace atlg {
+namespace {
+
+constexpr int kBW = 4;          // waves per workgroup
+constexpr int kBT = 64 * kBW;   // threads per workgroup
+constexpr int kTmax = 32;       // widest block column
+constexpr int kDP = kTmax + 1;  // LDS pitch of the diagonal block and inverse
+
+typedef double d4 __attribute__((ext_vector_type(4)));
+typedef float f4 __attribute__((ext_vector_type(4)));
+
+// The 16x16x4 MFMA for each scalar type.  D = X1 * X2 with lane l supplying
+// X1[l&15][l>>4] and X2[l>>4][l&15]; lane l, register r holds
+// D[row(l, r)][l & 15].
+template <class S>
+struct Mfma;
+template <>
+struct Mfma<double> {
+  typedef d4 acc_t;
+  static __device__ __forceinline__ acc_t op(double x1, double x2, acc_t c) {
+    return __builtin_amdgcn_mfma_f64_16x16x4f64(x1, x2, c, 0, 0, 0);
+  }
+  static __device__ __forceinline__ int row(int l, int r) { return (l >> 4) + 4 * r; }
+};
+template <>
+struct Mfma<float> {
+  typedef f4 acc_t;
+  static __device__ __forceinline__ acc_t op(float x1, float x2, acc_t c) {
+    return __builtin_amdgcn_mfma_f32_16x16x4f32(x1, x2, c, 0, 0, 0);
+  }
+  static __device__ __forceinline__ int row(int l, int r) { return 4 * (l >> 4) + r; }
+};
+
+template <class S>
+__device__ __forceinline__ S ssqrt(S x);
+template <>
+__device__ __forceinline__ double ssqrt<double>(double x) { return sqrt(x); }
+template <>
+__device__ __forceinline__ float ssqrt<float>(float x) { return sqrtf(x); }
+
+__device__ __forceinline__ void wave_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// LDS of one workgroup.  `D` and `V` are typed per scalar at use.
+struct alignas(16) Smem {
+  double D[kTmax * kDP];    // diagonal block (factor)
+  double V[kTmax * kDP];    // its inverse
+  double red[kBW * 4];      // reductions
+  int flag;                 // failure index + 1, 0 = fine
+  int cur;                  // system taken from the queue
+};
+
+// ---- U: C = A[J:n, J:J+w] - L[J:n, 0:J] L[J:J+w, 0:J]^T (lower part) -----
+template <class S, int T>
+__device__ void update_block(int n, S* __restrict__ A, int lda, int J, int w) {
+  typedef Mfma<S> M;
+  constexpr int CT = T / 16;
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int li = lane & 15, lk = lane >> 4;
+  const int RT = (n - J + 15) >> 4;
+  const S* pc[CT];
+  bool cv[CT];
+#pragma unroll
+  for (int ct = 0; ct < CT; ct++) {
+    const int col = J + 16 * ct + li;
+    cv[ct] = col < J + w;
+    pc[ct] = A + (cv[ct] ? col : 0) + (size_t)lk * lda;
+  }
+  for (int rt = wave; rt < RT; rt += kBW) {
+    const int row0 = J + 16 * rt;
+    const int row = row0 + li;
+    const bool rv = row < n;
+    const S* pr = A + (rv ? row : 0) + (size_t)lk * lda;
+    typename M::acc_t acc[CT];
+#pragma unroll
+    for (int ct = 0; ct < CT; ct++) acc[ct] = typename M::acc_t{0, 0, 0, 0};
+    // k = 4q + lk for q < J/4: every lane runs J/4 steps (J % 4 == 0)
+    const size_t step = (size_t)4 * lda;
+    size_t off = 0;
+#pragma unroll 4
+    for (int q = 0; q < (J >> 2); q++, off += step) {
+      const S x2 = rv ? pr[off] : S(0);
+#pragma unroll
+      for (int ct = 0; ct < CT; ct++) {
+        const S x1 = cv[ct] ? pc[ct][off] : S(0);
+        acc[ct] = M::op(x1, x2, acc[ct]);
+      }
+    }
+#pragma unroll
+    for (int ct = 0; ct < CT; ct++) {
+      if (row0 + 15 < J + 16 * ct) continue;  // tile wholly above the diagonal
+#pragma unroll
+      for (int r = 0; r < 4; r++) {
+        const int col = J + 16 * ct + M::row(lane, r);
+        if (rv && col < J + w && row >= col) A[row + (size_t)col * lda] -= acc[ct][r];
+      }
+    }
+  }
+}
+
+// ---- F: factor the w x w diagonal block (wave 0) and invert it ------------
+// Returns 0, or j+1 for the first non-positive pivot.  D (pitch kDP) receives
+// L_JJ, V its inverse, zero padded to T x T; L_JJ is written back to A.
+template <class S, int T>
+__device__ int factor_diag(int n, S* __restrict__ A, int lda, int J, int w, S* D, S* V) {
+  const int l = threadIdx.x;  // wave 0 only
+  for (int e = l; e < T * T; e += 64) {
+    const int i = e % T, c = e / T;
+    D[i * kDP + c] = (i < w && c <= i) ? A[(J + i) + (size_t)(J + c) * lda] : S(0);
+    V[i * kDP + c] = (i == c) ? S(1) : S(0);
+  }
+  wave_sync();
+  const int c = l & 31, h = l >> 5;
+  for (int j = 0; j < w; j++) {
+    const S d = D[j * kDP + j];
+    if (!(d > S(0))) return j + 1;  // d is the same in every lane
+    const S ljj = ssqrt(d), rinv = S(1) / ljj;
+    wave_sync();
+    if (l > j && l < w) D[l * kDP + j] *= rinv;
+    if (l == 0) D[j * kDP + j] = ljj;
+    wave_sync();
+    if (c > j)
+      for (int i = j + 1 + h; i < w; i += 2)
+        if (c <= i) D[i * kDP + c] -= D[i * kDP + j] * D[c * kDP + j];
+    wave_sync();
+  }
+  // Gauss-Jordan on [L | I]: V <- L^-1
+  for (int j = 0; j < w; j++) {
+    const S rinv = S(1) / D[j * kDP + j];
+    if (l <= j) V[j * kDP + l] *= rinv;
+    wave_sync();
+    if (c <= j)
+      for (int i = j + 1 + h; i < w; i += 2) V[i * kDP + c] -= D[i * kDP + j] * V[j * kDP + c];
+    wave_sync();
+  }
+  for (int e = l; e < w * w; e += 64) {
+    const int i = e % w, cc = e / w;
+    if (cc <= i) A[(J + i) + (size_t)(J + cc) * lda] = D[i * kDP + cc];
+  }
+  return 0;
+}
+
+// ---- T: A[J+w:n, J:J+w] <- A[J+w:n, J:J+w] * V^T ---------------------------
+template <class S, int T>
+__device__ void trsm_block(int n, S* __restrict__ A, int lda, int J, int w, const S* V) {
+  typedef Mfma<S> M;
+  constexpr int CT = T / 16, KQ = T / 4;
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int li = lane & 15, lk = lane >> 4;
+  const int r0 = J + w;
+  const int RT = (n - r0 + 15) >> 4;
+  for (int rt = wave; rt < RT; rt += kBW) {
+    const int row = r0 + 16 * rt + li;
+    const bool rv = row < n;
+    S x2[KQ];
+#pragma unroll
+    for (int q = 0; q < KQ; q++) {
+      const int k = 4 * q + lk;
+      x2[q] = (rv && k < w) ? A[row + (size_t)(J + k) * lda] : S(0);
+    }
+#pragma unroll
+    for (int ct = 0; ct < CT; ct++) {
+      typename M::acc_t acc = {0, 0, 0, 0};
+#pragma unroll
+      for (int q = 0; q < KQ; q++) acc = M::op(V[(16 * ct + li) * kDP + 4 * q + lk], x2[q], acc);
+#pragma unroll
+      for (int r = 0; r < 4; r++) {
+        const int col = 16 * ct + M::row(lane, r);
+        if (rv && col < w) A[row + (size_t)(J + col) * lda] = acc[r];
+      }
+    }
+  }
+}
+
+// Factors one system; returns 0 or the LAPACK info (first failing minor).
+template <class S, int T>
+__device__ int chol_sys(int n, S* __restrict__ A, int lda, Smem& sm) {
+  S* D = reinterpret_cast<S*>(sm.D);
+  S* V = reinterpret_cast<S*>(sm.V);
+  for (int J = 0; J < n; J += T) {
+    const int w = min(T, n - J);
+    if (J > 0) update_block<S, T>(n, A, lda, J, w);
+    __syncthreads();
+    if (threadIdx.x < 64) {
+      const int f = factor_diag<S, T>(n, A, lda, J, w, D, V);
+      if (threadIdx.x == 0) sm.flag = f ? J + f : 0;
+    }
+    __syncthreads();
+    const int f = sm.flag;
+    if (f) return f;
+    trsm_block<S, T>(n, A, lda, J, w, V);
+    __syncthreads();
+  }
+  return 0;
+}
+
+// Solves L L^T X = B in place (B: n x nrhs, column-major, ldb).
+template <class S, int T>
+__device__ void solve_sys(int n, const S* __restrict__ L, int lda, S* __restrict__ B, int ldb,
+                          int nrhs, Smem& sm) {
+  if (n <= 0) return;
+  S* D = reinterpret_cast<S*>(sm.D);
+  const int t = threadIdx.x, l = t & 63;
+  constexpr int RG = 64 / T;  // right-hand sides per wave pass
+  const int i = l % T, g = l / T;
+  // forward: L Y = B
+  for (int J = 0; J < n; J += T) {
+    const int w = min(T, n - J);
+    if (t < 64) {
+      for (int e = l; e < T * T; e += 64) {
+        const int a = e % T, c = e / T;
+        D[a * kDP + c] = (a < w && c <= a) ? L[(J + a) + (size_t)(J + c) * lda] : S(0);
+      }
+      wave_sync();
+      for (int r0 = 0; r0 < nrhs; r0 += RG) {
+        const int r = r0 + g;
+        const bool act = i < w && r < nrhs;
+        S y = act ? B[(J + i) + (size_t)r * ldb] : S(0);
+        for (int j = 0; j < w; j++) {
+          const S yj = __shfl(y, g * T + j, 64) / D[j * kDP + j];
+          if (i == j) y = yj;
+          else if (i > j) y -= D[i * kDP + j] * yj;
+        }
+        if (act) B[(J + i) + (size_t)r * ldb] = y;
+      }
+    }
+    __syncthreads();
+    for (int row = J + w + t; row < n; row += kBT) {
+      for (int r0 = 0; r0 < nrhs; r0 += 4) {
+        S acc[4] = {0, 0, 0, 0};
+        for (int c = 0; c < w; c++) {
+          const S lv = L[row + (size_t)(J + c) * lda];
+#pragma unroll
+          for (int q = 0; q < 4; q++)
+            if (r0 + q < nrhs) acc[q] += lv * B[(J + c) + (size_t)(r0 + q) * ldb];
+        }
+#pragma unroll
+        for (int q = 0; q < 4; q++)
+          if (r0 + q < nrhs) B[row + (size_t)(r0 + q) * ldb] -= acc[q];
+      }
+    }
+    __syncthreads();
+  }
+  // backward: L^T X = Y, last block row first
+  for (int J = ((n - 1) / T) * T; J >= 0; J -= T) {
+    const int w = min(T, n - J);
+    if (t < 64) {
+      for (int e = l; e < T * T; e += 64) {
+        const int a = e % T, c = e / T;
+        D[a * kDP + c] = (a < w && c <= a) ? L[(J + a) + (size_t)(J + c) * lda] : S(0);
+      }
+      wave_sync();
+      for (int r0 = 0; r0 < nrhs; r0 += RG) {
+        const int r = r0 + g;
+        const bool act = i < w && r < nrhs;
+        S x = act ? B[(J + i) + (size_t)r * ldb] : S(0);
+        for (int j = w - 1; j >= 0; j--) {
+          const S xj = __shfl(x, g * T + j, 64) / D[j * kDP + j];
+          if (i == j) x = xj;
+          else if (i < j) x -= D[j * kDP + i] * xj;
+        }
+        if (act) B[(J + i) + (size_t)r * ldb] = x;
+      }
+    }
+    __syncthreads();
+    for (int k = t; k < J; k += kBT) {
+      const S* lc = L + J + (size_t)k * lda;  // L[J:J+w, k], contiguous
+      for (int r0 = 0; r0 < nrhs; r0 += 4) {
+        S acc[4] = {0, 0, 0, 0};
+        for (int c = 0; c < w; c++) {
+          const S lv = lc[c];
+#pragma unroll
+          for (int q = 0; q < 4; q++)
+            if (r0 + q < nrhs) acc[q] += lv * B[(J + c) + (size_t)(r0 + q) * ldb];
+        }
+#pragma unroll
+        for (int q = 0; q < 4; q++)
+          if (r0 + q < nrhs) B[k + (size_t)(r0 + q) * ldb] -= acc[q];
+      }
+    }
+    __syncthreads();
+  }
+}
+
+template <class S>
+__device__ int chol_any(int n, S* A, int lda, int t16_max, Smem& sm) {
+  return (n <= t16_max) ? chol_sys<S, 16>(n, A, lda, sm) : chol_sys<S, 32>(n, A, lda, sm);
+}
+template <class S>
+__device__ void solve_any(int n, const S* L, int lda, S* B, int ldb, int nrhs, int t16_max,
+                          Smem& sm) {
+  if (n <= t16_max) solve_sys<S, 16>(n, L, lda, B, ldb, nrhs, sm);
+  else solve_sys<S, 32>(n, L, lda, B, ldb, nrhs, sm);
+}
+
+// Takes the next queue entry (or -1 when drained); every thread gets it.
+__device__ int next_system(unsigned* counter, int nsys, Smem& sm) {
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    const unsigned q = atomicAdd(counter, 1u);
+    sm.cur = (q < (unsigned)nsys) ? (int)q : -1;
+  }
+  __syncthreads();
+  return sm.cur;
+}
+
+template <class S>
+struct SysDesc {
+  S* A;
+  S* B;
+  const S* Bsrc;  // copied into B first when not null
+  int n, lda, ldb, idx;
+};
+
+// mode: 0 factor, 1 factor + solve, 2 solve with a given factor
+template <class S>
+__global__ void __launch_bounds__(kBT)
+posv_batched_kernel(int nsys, const SysDesc<S>* __restrict__ desc, int nrhs, int mode,
+                    int t16_max, int* __restrict__ info, unsigned* __restrict__ counter) {
+  __shared__ Smem sm;
+  for (;;) {
+    const int q = next_system(counter, nsys, sm);
+    if (q < 0) return;
+    const SysDesc<S> d = desc[q];
+    if (d.Bsrc) {
+      for (int r = 0; r < nrhs; r++)
+        for (int i = threadIdx.x; i < d.n; i += kBT)
+          d.B[i + (size_t)r * d.ldb] = d.Bsrc[i + (size_t)r * d.ldb];
+      __syncthreads();
+    }
+    int f = 0;
+    if (mode != 2) f = chol_any<S>(d.n, d.A, d.lda, t16_max, sm);
+    if (f == 0 && mode != 0) solve_any<S>(d.n, d.A, d.lda, d.B, d.ldb, nrhs, t16_max, sm);
+    if (threadIdx.x == 0 && mode != 2) info[d.idx] = f;
+  }
+}
+
+// ---- mixed precision: FP32 factor, FP64 residual and update ---------------
+struct MixDesc {
+  const double* A;  // FP64 matrix (lower triangle read)
+  const double* B;  // FP64 right-hand sides
+  double* X;        // FP64 solution
+  float* A32;       // workspace n x n (ld n)
+  float* W32;       // workspace n x nrhs (ld n)
+  int n, lda, ldb, ldx, idx, pad;
+};
+
+__device__ double block_max(double v, Smem& sm) {
+  for (int o = 32; o > 0; o >>= 1) v = fmax(v, __shfl_xor(v, o, 64));
+  __syncthreads();
+  if ((threadIdx.x & 63) == 0) sm.red[threadIdx.x >> 6] = v;
+  __syncthreads();
+  double m = sm.red[0];
+  for (int k = 1; k < kBW; k++) m = fmax(m, sm.red[k]);
+  return m;
+}
+
+// r = B - A X for the rows this thread owns; returns max |x| and max |r| of
+// right-hand side r0..r0+3 across the workgroup, writing float(r) to W32.
+__device__ void residual(const MixDesc& d, int nrhs, Smem& sm, bool* conv, double cte) {
+  const int n = d.n;
+  for (int r0 = 0; r0 < nrhs; r0 += 4) {
+    double xm[4] = {0, 0, 0, 0}, rm[4] = {0, 0, 0, 0};
+    for (int i = threadIdx.x; i < n; i += kBT) {
+      double acc[4] = {0, 0, 0, 0};
+      for (int j = 0; j <= i; j++) {  // row i of the lower triangle
+        const double a = d.A[i + (size_t)j * d.lda];
+#pragma unroll
+        for (int q = 0; q < 4; q++)
+          if (r0 + q < nrhs) acc[q] += a * d.X[j + (size_t)(r0 + q) * d.ldx];
+      }
+      const double* col = d.A + (size_t)i * d.lda;
+      for (int j = i + 1; j < n; j++) {  // column i below the diagonal
+        const double a = col[j];
+#pragma unroll
+        for (int q = 0; q < 4; q++)
+          if (r0 + q < nrhs) acc[q] += a * d.X[j + (size_t)(r0 + q) * d.ldx];
+      }
+#pragma unroll
+      for (int q = 0; q < 4; q++)
+        if (r0 + q < nrhs) {
+          const double rv = d.B[i + (size_t)(r0 + q) * d.ldb] - acc[q];
+          xm[q] = fmax(xm[q], fabs(d.X[i + (size_t)(r0 + q) * d.ldx]));
+          rm[q] = fmax(rm[q], fabs(rv));
+          d.W32[i + (size_t)(r0 + q) * n] = (float)rv;
+        }
+    }
+#pragma unroll
+    for (int q = 0; q < 4; q++) {
+      if (r0 + q >= nrhs) break;
+      const double xn = block_max(xm[q], sm), rn = block_max(rm[q], sm);
+      if (!(rn < xn * cte)) *conv = false;
+    }
+  }
+}
+
+__global__ void __launch_bounds__(kBT)
+mixed_posv_kernel(int nsys, const MixDesc* __restrict__ desc, int nrhs, int itermax, int t16_max,
+                  int* __restrict__ info, int* __restrict__ iter, unsigned* __restrict__ counter) {
+  __shared__ Smem sm;
+  for (;;) {
+    const int q = next_system(counter, nsys, sm);
+    if (q < 0) return;
+    const MixDesc d = desc[q];
+    const int n = d.n;
+    // ||A||_inf of the symmetric matrix, A32 = float(lower(A)), overflow check
+    double rs = 0.0;
+    bool over = false;
+    for (int i = threadIdx.x; i < n; i += kBT) {
+      double s = 0.0;
+      for (int j = 0; j <= i; j++) {
+        const double a = d.A[i + (size_t)j * d.lda];
+        s += fabs(a);
+        if (fabs(a) > (double)FLT_MAX) over = true;
+        d.A32[i + (size_t)j * n] = (float)a;
+      }
+      const double* col = d.A + (size_t)i * d.lda;
+      for (int j = i + 1; j < n; j++) s += fabs(col[j]);
+      rs = fmax(rs, s);
+    }
+    const double anrm = block_max(rs, sm);
+    const double ov = block_max(over ? 1.0 : 0.0, sm);
+    int it = 0;
+    if (n == 0) {
+      it = 0;
+    } else if (ov != 0.0) {
+      it = -2;
+    } else if (chol_any<float>(n, d.A32, n, t16_max, sm) != 0) {
+      it = -3;
+    } else {
+      // X = float solve of B
+      for (int r = 0; r < nrhs; r++)
+        for (int i = threadIdx.x; i < n; i += kBT)
+          d.W32[i + (size_t)r * n] = (float)d.B[i + (size_t)r * d.ldb];
+      __syncthreads();
+      solve_any<float>(n, d.A32, n, d.W32, n, nrhs, t16_max, sm);
+      for (int r = 0; r < nrhs; r++)
+        for (int i = threadIdx.x; i < n; i += kBT)
+          d.X[i + (size_t)r * d.ldx] = (double)d.W32[i + (size_t)r * n];
+      __syncthreads();
+      const double cte = anrm * (DBL_EPSILON * 0.5) * sqrt((double)n);
+      it = -(itermax + 1);
+      for (int k = 0; k <= itermax; k++) {
+        bool conv = true;
+        residual(d, nrhs, sm, &conv, cte);  // ends with a barrier (block_max)
+        if (conv) {
+          it = k;
+          break;
+        }
+        if (k == itermax) break;
+        solve_any<float>(n, d.A32, n, d.W32, n, nrhs, t16_max, sm);
+        for (int r = 0; r < nrhs; r++)
+          for (int i = threadIdx.x; i < n; i += kBT)
+            d.X[i + (size_t)r * d.ldx] += (double)d.W32[i + (size_t)r * n];
+        __syncthreads();
+      }
+    }
+    if (threadIdx.x == 0) {
+      iter[d.idx] = it;
+      info[d.idx] = 0;
+    }
+  }
+}
+
+__global__ void zero_counter(unsigned* c) { *c = 0u; }
+
+// ---- host side -------------------------------------------------------------
+
+// Pinned staging for the descriptor upload, reused once the previous copy
+// from it has completed.
+struct Staging {
+  void* host = nullptr;
+  size_t cap = 0;
+  hipEvent_t done = nullptr;
+};
+std::mutex g_stage_mu;
+Staging g_stage;
+
+void* stage_upload(hipStream_t s, const void* src, size_t bytes, Slot slot) {
+  void* dev = scratch(s, slot, bytes);
+  std::lock_guard<std::mutex> lk(g_stage_mu);
+  if (g_stage.done) ATLG_HIP(hipEventSynchronize(g_stage.done));
+  if (bytes > g_stage.cap) {
+    if (g_stage.host) ATLG_HIP(hipHostFree(g_stage.host));
+    g_stage.host = nullptr;
+    g_stage.cap = 0;
+    ATLG_HIP(hipHostMalloc(&g_stage.host, bytes, hipHostMallocDefault));
+    g_stage.cap = bytes;
+  }
+  if (!g_stage.done) ATLG_HIP(hipEventCreateWithFlags(&g_stage.done, hipEventDisableTiming));
+  std::memcpy(g_stage.host, src, bytes);
+  ATLG_HIP(hipMemcpyAsync(dev, g_stage.host, bytes, hipMemcpyHostToDevice, s));
+  ATLG_HIP(hipEventRecord(g_stage.done, s));
+  return dev;
+}
+
+// Longest-first order of the systems (n^3 decides; ties by index).
+std::vector<int> lpt_order(int nsys, const int* n) {
+  std::vector<int> ord(nsys);
+  std::iota(ord.begin(), ord.end(), 0);
+  std::stable_sort(ord.begin(), ord.end(), [n](int a, int b) { return n[a] > n[b]; });
+  return ord;
+}
+
+int grid_for(hipStream_t s, const BatchTune& tune, int nsys) {
+  (void)s;
+  int dev = 0, cus = 256;
+  ATLG_HIP(hipGetDevice(&dev));
+  ATLG_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
+  return std::max(1, std::min(nsys, cus * std::max(1, tune.wg_per_cu)));
+}
+
+unsigned* reset_counter(hipStream_t s) {
+  unsigned* counter = static_cast<unsigned*>(scratch(s, kSlotBatchCtr, 256));
+  hipLaunchKernelGGL(zero_counter, dim3(1), dim3(1), 0, s, counter);
+  ATLG_LAUNCHED();
+  return counter;
+}
+
+BatchTune g_tune[2] = {ATLG_BATCH_TUNE_D, ATLG_BATCH_TUNE_S};
+std::mutex g_tune_mu;
+
+}  // namespace
+
+BatchTune batch_tune(bool single) {
+  std::lock_guard<std::mutex> lk(g_tune_mu);
+  return g_tune[single ? 1 : 0];
+}
+void set_batch_tune(bool single, const BatchTune& t) {
+  std::lock_guard<std::mutex> lk(g_tune_mu);
+  g_tune[single ? 1 : 0] = t;
+}
+
+template <class S>
+void posv_batched(hipStream_t s, int mode, int nsys, const int* n, S* const* A, const int* lda,
+                  S* const* B, const int* ldb, const S* const* Bsrc, int nrhs, int* info,
+                  const BatchTune& tune, const int* orig) {
+  if (nsys <= 0) return;
+  const std::vector<int> ord = lpt_order(nsys, n);
+  std::vector<SysDesc<S>> desc(nsys);
+  for (int q = 0; q < nsys; q++) {
+    const int k = ord[q];
+    desc[q] = SysDesc<S>{A[k], B ? B[k] : nullptr, Bsrc ? Bsrc[k] : nullptr, n[k], lda[k],
+                         ldb ? ldb[k] : 1, orig ? orig[k] : k};
+  }
+  auto* dd = static_cast<const SysDesc<S>*>(
+      stage_upload(s, desc.data(), sizeof(SysDesc<S>) * nsys, kSlotBatchDesc));
+  unsigned* counter = reset_counter(s);
+  hipLaunchKernelGGL(posv_batched_kernel<S>, dim3(grid_for(s, tune, nsys)), dim3(kBT), 0, s, nsys,
+                     dd, nrhs, mode, tune.t16_max, info, counter);
+  ATLG_LAUNCHED();
+}
+
+template void posv_batched<double>(hipStream_t, int, int, const int*, double* const*, const int*,
+                                   double* const*, const int*, const double* const*, int, int*,
+                                   const BatchTune&, const int*);
+template void posv_batched<float>(hipStream_t, int, int, const int*, float* const*, const int*,
+                                  float* const*, const int*, const float* const*, int, int*,
+                                  const BatchTune&, const int*);
+
+void dsposv_batched(hipStream_t s, int nsys, const int* n, double* const* A, const int* lda,
+                    const double* const* B, const int* ldb, double* const* X, const int* ldx,
+                    int nrhs, int* info, int* iter, int itermax, const BatchTune& tune) {
+  if (nsys <= 0) return;
+  const std::vector<int> ord = lpt_order(nsys, n);
+  size_t ws = 0;
+  std::vector<size_t> off(nsys);
+  for (int q = 0; q < nsys; q++) {
+    off[q] = ws;
+    const size_t nn = (size_t)n[ord[q]];
+    ws += (nn * nn + nn * (size_t)nrhs + 63) & ~(size_t)63;  // floats, 256-byte aligned
+  }
+  float* W = static_cast<float*>(scratch(s, kSlotBatchWork, std::max<size_t>(ws, 64) * 4));
+  std::vector<MixDesc> desc(nsys);
+  for (int q = 0; q < nsys; q++) {
+    const int k = ord[q];
+    const size_t nn = (size_t)n[k];
+    desc[q] = MixDesc{A[k], B[k], X[k], W + off[q], W + off[q] + nn * nn, n[k], lda[k], ldb[k],
+                      ldx[k], k, 0};
+  }
+  auto* dd = static_cast<const MixDesc*>(
+      stage_upload(s, desc.data(), sizeof(MixDesc) * nsys, kSlotBatchDesc));
+  unsigned* counter = reset_counter(s);
+  hipLaunchKernelGGL(mixed_posv_kernel, dim3(grid_for(s, tune, nsys)), dim3(kBT), 0, s, nsys, dd,
+                     nrhs, itermax, tune.t16_max, info, iter, counter);
+  ATLG_LAUNCHED();
+  // Systems whose refinement did not converge (or whose FP32 factor failed)
+  // get the FP64 factor and solve, as LAPACK's dsposv does; A then holds the
+  // FP64 factor and X the FP64 solution.
+  std::vector<int> it(nsys);
+  ATLG_HIP(hipMemcpyAsync(it.data(), iter, sizeof(int) * nsys, hipMemcpyDeviceToHost, s));
+  ATLG_HIP(hipStreamSynchronize(s));
+  std::vector<int> fn, flda, fldx, forig;
+  std::vector<double*> fA, fX;
+  std::vector<const double*> fB;
+  for (int k = 0; k < nsys; k++) {
+    if (it[k] >= 0) continue;
+    fn.push_back(n[k]);
+    flda.push_back(lda[k]);
+    fldx.push_back(ldx[k]);
+    forig.push_back(k);
+    fA.push_back(A[k]);
+    fX.push_back(X[k]);
+    if (ldb[k] == ldx[k]) {
+      fB.push_back(B[k]);  // copied into X inside the launch
+    } else {
+      ATLG_HIP(hipMemcpy2DAsync(X[k], sizeof(double) * ldx[k], B[k], sizeof(double) * ldb[k],
+                                sizeof(double) * n[k], nrhs, hipMemcpyDeviceToDevice, s));
+      fB.push_back(nullptr);
+    }
+  }
+  if (fn.empty()) return;
+  posv_batched<double>(s, 1, (int)fn.size(), fn.data(), fA.data(), flda.data(), fX.data(),
+                       fldx.data(), fB.data(), nrhs, info, batch_tune(false), forig.data());
+  ATLG_HIP(hipStreamSynchronize(s));
+}
+
+}  // namespace atlg

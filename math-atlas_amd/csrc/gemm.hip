@@ -1,4 +1,6 @@
-// DGEMM family for gfx950 on the FP64 matrix cores (v_mfma_f64_16x16x4_f64).
+// GEMM family for gfx950 on the matrix cores: FP64 (v_mfma_f64_16x16x4_f64)
+// and, with the same tiling, FP32 (v_mfma_f32_16x16x4_f32) for the
+// single-precision factorization of the mixed-precision solver.
 //
 // Replaces the work ATLAS does in ATL_dgemm/ATL_dammm (recursion + shape
 // dispatch, AtlasBase/Clint/atlas-ammm.base:17266-17810), its copy-to-block
@@ -21,92 +23,107 @@
 namespace atlg {
 namespace {
 
-typedef double d4 __attribute__((ext_vector_type(4)));
-
 constexpr int BM = 128, BN = 128, BK = 16, NT = 256;
 constexpr int WM = 64, WN = 64;           // wave tile
 constexpr int TM = WM / 16, TN = WN / 16; // MFMA tiles per wave
-constexpr int PADA = 4, PADB = 4;         // LDS row padding (doubles)
+constexpr int PADA = 4, PADB = 4;         // LDS row padding (elements)
 constexpr int LDA_S = BM + PADA, LDB_S = BN + PADB;
 
-// Register layout of v_mfma_f64_16x16x4_f64 (D = A*B + C, A 16x4, B 4x16),
-// measured on gfx950 with one-hot operands: lane l supplies A[l&15][l>>4]
-// and B[l>>4][l&15]; D[i][j] is in lane j + 16*(i&3), register i>>2.
-// The kernel passes the B tile as the first operand and the A tile as the
-// second, so the accumulator holds C^T: lane l, register r carries
-// C[m = l&15][n = (l>>4) + 4r], and a register's 16 lanes of one row group
-// are 16 consecutive rows of a column (one 128-byte store).
-struct MfmaF64 {
-  __device__ static int frag_x(int l) { return l & 15; }   // m (or n) of a fragment
-  __device__ static int frag_k(int l) { return l >> 4; }   // k of a fragment
-  __device__ static int c_m(int l) { return l & 15; }
+// The 16x16x4 MFMA of each precision (D = X1*X2 + C, X1 16x4, X2 4x16).
+// Operand layout (both precisions): lane l supplies X1[l&15][l>>4] and
+// X2[l>>4][l&15].  Result layout: FP64 keeps D[i][j] in lane j + 16*(i&3),
+// register i>>2 (measured on gfx950 with one-hot operands); FP32 keeps it in
+// lane j + 16*(i>>2), register i&3 (the CDNA layout of the single-block
+// 16x16 FP32 MFMA, confirmed by the SGEMM parity tests).  The kernel passes the B tile as X1 and the A tile as X2, so the
+// accumulator holds C^T: lane l, register r carries C[m = l&15][n = c_n(l, r)],
+// and a register's 16 lanes of one row group are 16 consecutive rows of a
+// column (one coalesced store).
+template <class S>
+struct Mfma;
+template <>
+struct Mfma<double> {
+  typedef double acc_t __attribute__((ext_vector_type(4)));
+  __device__ static acc_t op(double x1, double x2, acc_t c) {
+    return __builtin_amdgcn_mfma_f64_16x16x4f64(x1, x2, c, 0, 0, 0);
+  }
   __device__ static int c_n(int l, int r) { return (l >> 4) + 4 * r; }
 };
+template <>
+struct Mfma<float> {
+  typedef float acc_t __attribute__((ext_vector_type(4)));
+  __device__ static acc_t op(float x1, float x2, acc_t c) {
+    return __builtin_amdgcn_mfma_f32_16x16x4f32(x1, x2, c, 0, 0, 0);
+  }
+  __device__ static int c_n(int l, int r) { return 4 * (l >> 4) + r; }
+};
+__device__ inline int frag_x(int l) { return l & 15; }  // m (or n) of a fragment
+__device__ inline int frag_k(int l) { return l >> 4; }  // k of a fragment
+__device__ inline int c_m(int l) { return l & 15; }
 
-// Loads one BK x 128 operand tile into registers.  Element (x, k) of the
-// tile (x = m for A, n for B) lives at P[x + k*ld] when XCONTIG (A NoTrans,
-// B Trans), else at P[k + x*ld] (A Trans, B NoTrans).
-template <bool XCONTIG, bool VEC>
+// Loads one BK x 128 operand tile into registers (8 elements per thread).
+// Element (x, k) of the tile (x = m for A, n for B) lives at P[x + k*ld] when
+// XCONTIG (A NoTrans, B Trans), else at P[k + x*ld] (A Trans, B NoTrans).
+// VEC: the 8 elements are read with 16-byte loads.
+template <class S, bool XCONTIG, bool VEC>
 struct TileLoader {
-  double v[8];
-  __device__ void load(const double* __restrict__ P, long ld, int x0, int k0,
-                       int Xtot, int Ktot, int t) {
+  static constexpr int VW = 16 / sizeof(S);  // elements per 16-byte vector
+  typedef S vec_t __attribute__((ext_vector_type(VW)));
+  S v[8];
+  __device__ void load8(const S* p) {
+    if (VEC) {
+      const vec_t* q = reinterpret_cast<const vec_t*>(p);
+#pragma unroll
+      for (int i = 0; i < 8 / VW; i++) {
+        const vec_t w = q[i];
+#pragma unroll
+        for (int e = 0; e < VW; e++) v[VW * i + e] = w[e];
+      }
+    } else {
+#pragma unroll
+      for (int i = 0; i < 8; i++) v[i] = p[i];
+    }
+  }
+  __device__ void load(const S* __restrict__ P, long ld, int x0, int k0, int Xtot, int Ktot,
+                       int t) {
     if (XCONTIG) {
       const int k = t >> 4, x = (t & 15) * 8;
       const int gk = k0 + k, gx = x0 + x;
-      const double* p = P + (size_t)gx + (size_t)gk * ld;
+      const S* p = P + (size_t)gx + (size_t)gk * ld;
       if (gk < Ktot && gx + 8 <= Xtot) {
-        if (VEC) {
-          const double2* q = reinterpret_cast<const double2*>(p);
-#pragma unroll
-          for (int i = 0; i < 4; i++) {
-            double2 w = q[i];
-            v[2 * i] = w.x;
-            v[2 * i + 1] = w.y;
-          }
-        } else {
-#pragma unroll
-          for (int i = 0; i < 8; i++) v[i] = p[i];
-        }
+        load8(p);
       } else {
 #pragma unroll
-        for (int i = 0; i < 8; i++) v[i] = (gk < Ktot && gx + i < Xtot) ? p[i] : 0.0;
+        for (int i = 0; i < 8; i++) v[i] = (gk < Ktot && gx + i < Xtot) ? p[i] : S(0);
       }
     } else {
       const int x = t >> 1, k = (t & 1) * 8;
       const int gk = k0 + k, gx = x0 + x;
-      const double* p = P + (size_t)gk + (size_t)gx * ld;
+      const S* p = P + (size_t)gk + (size_t)gx * ld;
       if (gx < Xtot && gk + 8 <= Ktot) {
-        if (VEC) {
-          const double2* q = reinterpret_cast<const double2*>(p);
-#pragma unroll
-          for (int i = 0; i < 4; i++) {
-            double2 w = q[i];
-            v[2 * i] = w.x;
-            v[2 * i + 1] = w.y;
-          }
-        } else {
-#pragma unroll
-          for (int i = 0; i < 8; i++) v[i] = p[i];
-        }
+        load8(p);
       } else {
 #pragma unroll
-        for (int i = 0; i < 8; i++) v[i] = (gx < Xtot && gk + i < Ktot) ? p[i] : 0.0;
+        for (int i = 0; i < 8; i++) v[i] = (gx < Xtot && gk + i < Ktot) ? p[i] : S(0);
       }
     }
   }
   // LDS tile is [BK][LD] with x contiguous.
   template <int LD>
-  __device__ void store(double* S, int t) const {
+  __device__ void store(S* T, int t) const {
     if (XCONTIG) {
       const int k = t >> 4, x = (t & 15) * 8;
-      double2* q = reinterpret_cast<double2*>(S + k * LD + x);
+      vec_t* q = reinterpret_cast<vec_t*>(T + k * LD + x);
 #pragma unroll
-      for (int i = 0; i < 4; i++) q[i] = make_double2(v[2 * i], v[2 * i + 1]);
+      for (int i = 0; i < 8 / VW; i++) {
+        vec_t w;
+#pragma unroll
+        for (int e = 0; e < VW; e++) w[e] = v[VW * i + e];
+        q[i] = w;
+      }
     } else {
       const int x = t >> 1, k = (t & 1) * 8;
 #pragma unroll
-      for (int i = 0; i < 8; i++) S[(k + i) * LD + x] = v[i];
+      for (int i = 0; i < 8; i++) T[(k + i) * LD + x] = v[i];
     }
   }
 };
@@ -129,11 +146,11 @@ __device__ inline void tile_of(int bid, int nblk, int tilesM, int tilesN, int& t
   tn = in / gsz;
 }
 
-template <bool AT, bool BT, bool VA, bool VB>
+template <class S, bool AT, bool BT, bool VA, bool VB>
 __global__ void __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(2, 2)))
-dgemm_mfma(int M, int N, int K, double alpha, const double* A, long lda,
-           const double* B, long ldb, double beta, double* C, long ldc, int mode,
-           const int* __restrict__ skip) {
+gemm_mfma(int M, int N, int K, S alpha, const S* A, long lda, const S* B, long ldb, S beta,
+          S* C, long ldc, int mode, const int* __restrict__ skip) {
+  typedef Mfma<S> MF;
   if (skip && *skip) return;
   const int tilesM = (M + BM - 1) / BM, tilesN = (N + BN - 1) / BN;
   int tm, tn;
@@ -142,22 +159,22 @@ dgemm_mfma(int M, int N, int K, double alpha, const double* A, long lda,
   if (mode == 1 && n0 >= m0 + BM) return;  // tile strictly above the diagonal
   if (mode == 2 && m0 >= n0 + BN) return;  // tile strictly below the diagonal
 
-  __shared__ __attribute__((aligned(16))) double As[2][BK * LDA_S];
-  __shared__ __attribute__((aligned(16))) double Bs[2][BK * LDB_S];
+  __shared__ __attribute__((aligned(16))) S As[2][BK * LDA_S];
+  __shared__ __attribute__((aligned(16))) S Bs[2][BK * LDB_S];
 
   const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
   const int wm0 = (wave >> 1) * WM, wn0 = (wave & 1) * WN;
 
   // A tile: x = m; contiguous in m unless transposed.  B tile: x = n;
   // contiguous in n only when transposed.
-  TileLoader<!AT, VA> la;
-  TileLoader<BT, VB> lb;
+  TileLoader<S, !AT, VA> la;
+  TileLoader<S, BT, VB> lb;
 
-  d4 acc[TM][TN];
+  typename MF::acc_t acc[TM][TN];
 #pragma unroll
   for (int i = 0; i < TM; i++)
 #pragma unroll
-    for (int j = 0; j < TN; j++) acc[i][j] = d4{0.0, 0.0, 0.0, 0.0};
+    for (int j = 0; j < TN; j++) acc[i][j] = typename MF::acc_t{0, 0, 0, 0};
 
   const int nk = (K + BK - 1) / BK;
   la.load(A, lda, m0, 0, M, K, t);
@@ -166,7 +183,7 @@ dgemm_mfma(int M, int N, int K, double alpha, const double* A, long lda,
   lb.template store<LDB_S>(Bs[0], t);
   __syncthreads();
 
-  const int fx = MfmaF64::frag_x(lane), fk = MfmaF64::frag_k(lane);
+  const int fx = frag_x(lane), fk = frag_k(lane);
 
   for (int kt = 0; kt < nk; kt++) {
     const int cur = kt & 1;
@@ -175,11 +192,11 @@ dgemm_mfma(int M, int N, int K, double alpha, const double* A, long lda,
       la.load(A, lda, m0, (kt + 1) * BK, M, K, t);
       lb.load(B, ldb, n0, (kt + 1) * BK, N, K, t);
     }
-    const double* as = As[cur];
-    const double* bs = Bs[cur];
+    const S* as = As[cur];
+    const S* bs = Bs[cur];
 #pragma unroll
     for (int ks = 0; ks < BK / 4; ks++) {
-      double af[TM], bf[TN];
+      S af[TM], bf[TN];
 #pragma unroll
       for (int i = 0; i < TM; i++) af[i] = as[(ks * 4 + fk) * LDA_S + wm0 + i * 16 + fx];
 #pragma unroll
@@ -187,8 +204,7 @@ dgemm_mfma(int M, int N, int K, double alpha, const double* A, long lda,
 #pragma unroll
       for (int i = 0; i < TM; i++)
 #pragma unroll
-        for (int j = 0; j < TN; j++)
-          acc[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(bf[j], af[i], acc[i][j], 0, 0, 0);
+        for (int j = 0; j < TN; j++) acc[i][j] = MF::op(bf[j], af[i], acc[i][j]);
     }
     if (more) {
       la.template store<LDA_S>(As[cur ^ 1], t);
@@ -205,18 +221,19 @@ dgemm_mfma(int M, int N, int K, double alpha, const double* A, long lda,
     for (int j = 0; j < TN; j++)
 #pragma unroll
       for (int r = 0; r < 4; r++) {
-        const int gi = m0 + wm0 + i * 16 + MfmaF64::c_m(lane);
-        const int gj = n0 + wn0 + j * 16 + MfmaF64::c_n(lane, r);
+        const int gi = m0 + wm0 + i * 16 + c_m(lane);
+        const int gj = n0 + wn0 + j * 16 + MF::c_n(lane, r);
         if (gi >= M || gj >= N) continue;
         if (mode == 1 && gi < gj) continue;
         if (mode == 2 && gi > gj) continue;
-        double* c = C + (size_t)gi + (size_t)gj * ldc;
-        const double v = alpha * acc[i][j][r];
-        *c = (beta == 0.0) ? v : v + beta * (*c);
+        S* c = C + (size_t)gi + (size_t)gj * ldc;
+        const S v = alpha * acc[i][j][r];
+        *c = (beta == S(0)) ? v : v + beta * (*c);
       }
 }
 
-__global__ void scale_kernel(int M, int N, double beta, double* C, long ldc, int mode,
+template <class S>
+__global__ void scale_kernel(int M, int N, S beta, S* C, long ldc, int mode,
                              const int* __restrict__ skip) {
   if (skip && *skip) return;
   const long total = (long)M * N;
@@ -225,8 +242,8 @@ __global__ void scale_kernel(int M, int N, double beta, double* C, long ldc, int
     const int i = (int)(idx % M), j = (int)(idx / M);
     if (mode == 1 && i < j) continue;
     if (mode == 2 && i > j) continue;
-    double* c = C + (size_t)i + (size_t)j * ldc;
-    *c = (beta == 0.0) ? 0.0 : beta * (*c);
+    S* c = C + (size_t)i + (size_t)j * ldc;
+    *c = (beta == S(0)) ? S(0) : beta * (*c);
   }
 }
 
@@ -241,13 +258,12 @@ __global__ void syreflect_kernel(int N, double* C, long ldc) {
   }
 }
 
-template <bool AT, bool BT>
-void launch_t(hipStream_t s, dim3 g, bool va, bool vb, int M, int N, int K, double alpha,
-              const double* A, long lda, const double* B, long ldb, double beta, double* C,
-              long ldc, int mode, const int* skip) {
-#define ATLG_GO(VA_, VB_)                                                          \
-  hipLaunchKernelGGL((dgemm_mfma<AT, BT, VA_, VB_>), g, dim3(NT), 0, s, M, N, K, alpha, \
-                     A, lda, B, ldb, beta, C, ldc, mode, skip)
+template <class S, bool AT, bool BT>
+void launch_t(hipStream_t s, dim3 g, bool va, bool vb, int M, int N, int K, S alpha, const S* A,
+              long lda, const S* B, long ldb, S beta, S* C, long ldc, int mode, const int* skip) {
+#define ATLG_GO(VA_, VB_)                                                                  \
+  hipLaunchKernelGGL((gemm_mfma<S, AT, BT, VA_, VB_>), g, dim3(NT), 0, s, M, N, K, alpha, A, \
+                     lda, B, ldb, beta, C, ldc, mode, skip)
   if (va && vb) ATLG_GO(true, true);
   else if (va) ATLG_GO(true, false);
   else if (vb) ATLG_GO(false, true);
@@ -255,18 +271,19 @@ void launch_t(hipStream_t s, dim3 g, bool va, bool vb, int M, int N, int K, doub
 #undef ATLG_GO
 }
 
+template <class S>
 bool aligned16(const void* p, long ld) {
-  return (reinterpret_cast<uintptr_t>(p) % 16 == 0) && (ld % 2 == 0);
+  return (reinterpret_cast<uintptr_t>(p) % 16 == 0) && ((ld * (long)sizeof(S)) % 16 == 0);
 }
 
 }  // namespace
 
-void scale(hipStream_t s, long M, long N, double beta, double* C, long ldc, int mode,
-           const int* skip) {
-  if (M <= 0 || N <= 0 || beta == 1.0) return;
+template <class S>
+void scale(hipStream_t s, long M, long N, S beta, S* C, long ldc, int mode, const int* skip) {
+  if (M <= 0 || N <= 0 || beta == S(1)) return;
   const long total = M * N;
   const int nb = (int)std::min<long>((total + 255) / 256, 4096);
-  hipLaunchKernelGGL(scale_kernel, dim3(nb), dim3(256), 0, s, (int)M, (int)N, beta, C, ldc,
+  hipLaunchKernelGGL(scale_kernel<S>, dim3(nb), dim3(256), 0, s, (int)M, (int)N, beta, C, ldc,
                      mode, skip);
   ATLG_LAUNCHED();
 }
@@ -278,27 +295,38 @@ void syreflect_upper(hipStream_t s, long N, double* C, long ldc) {
   ATLG_LAUNCHED();
 }
 
-void gemm(hipStream_t s, int TA, int TB, long M, long N, long K, double alpha,
-          const double* A, long lda, const double* B, long ldb, double beta, double* C,
-          long ldc, int mode, const int* skip) {
+template <class S>
+void gemm(hipStream_t s, int TA, int TB, long M, long N, long K, S alpha, const S* A, long lda,
+          const S* B, long ldb, S beta, S* C, long ldc, int mode, const int* skip) {
   if (M <= 0 || N <= 0) return;
-  if (K == 0 || alpha == 0.0) {
-    scale(s, M, N, beta, C, ldc, mode, skip);
+  if (K == 0 || alpha == S(0)) {
+    scale<S>(s, M, N, beta, C, ldc, mode, skip);
     return;
   }
   const long tilesM = (M + BM - 1) / BM, tilesN = (N + BN - 1) / BN;
   const dim3 g((unsigned)(tilesM * tilesN));
   const bool at = is_trans(TA), bt = is_trans(TB);
-  const bool va = aligned16(A, lda), vb = aligned16(B, ldb);
+  const bool va = aligned16<S>(A, lda), vb = aligned16<S>(B, ldb);
   if (!at && !bt)
-    launch_t<false, false>(s, g, va, vb, M, N, K, alpha, A, lda, B, ldb, beta, C, ldc, mode, skip);
+    launch_t<S, false, false>(s, g, va, vb, M, N, K, alpha, A, lda, B, ldb, beta, C, ldc, mode,
+                              skip);
   else if (!at && bt)
-    launch_t<false, true>(s, g, va, vb, M, N, K, alpha, A, lda, B, ldb, beta, C, ldc, mode, skip);
+    launch_t<S, false, true>(s, g, va, vb, M, N, K, alpha, A, lda, B, ldb, beta, C, ldc, mode,
+                             skip);
   else if (at && !bt)
-    launch_t<true, false>(s, g, va, vb, M, N, K, alpha, A, lda, B, ldb, beta, C, ldc, mode, skip);
+    launch_t<S, true, false>(s, g, va, vb, M, N, K, alpha, A, lda, B, ldb, beta, C, ldc, mode,
+                             skip);
   else
-    launch_t<true, true>(s, g, va, vb, M, N, K, alpha, A, lda, B, ldb, beta, C, ldc, mode, skip);
+    launch_t<S, true, true>(s, g, va, vb, M, N, K, alpha, A, lda, B, ldb, beta, C, ldc, mode,
+                            skip);
   ATLG_LAUNCHED();
 }
+
+template void scale<double>(hipStream_t, long, long, double, double*, long, int, const int*);
+template void scale<float>(hipStream_t, long, long, float, float*, long, int, const int*);
+template void gemm<double>(hipStream_t, int, int, long, long, long, double, const double*, long,
+                           const double*, long, double, double*, long, int, const int*);
+template void gemm<float>(hipStream_t, int, int, long, long, long, float, const float*, long,
+                          const float*, long, float, float*, long, int, const int*);
 
 }  // namespace atlg

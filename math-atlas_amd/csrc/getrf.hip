@@ -28,11 +28,16 @@
 // Pivot choice follows cblas_idamax (AtlasBase/Antoine/atlrefblas.base:753+):
 // first index of the largest |a|, NaN never chosen; the column is scaled by
 // the reciprocal pivot when |pivot| >= SAFMIN, else divided.
+//
+// Everything is a template on the scalar type: double for ATL_dgetrf, float
+// for the single-precision factor of the mixed-precision solver (the
+// ATL_sgetrf instance of the same ATLAS source).
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
 #include <cfloat>
 #include <climits>
+#include <limits>
 #include <vector>
 
 #include "atlg_internal.hpp"
@@ -56,7 +61,8 @@ struct Cand {
   int pad;
 };
 
-__device__ inline double pkey(double v) { return (v != v) ? -1.0 : fabs(v); }
+template <class S>
+__device__ inline double pkey(S v) { return (v != v) ? -1.0 : fabs((double)v); }
 
 __device__ inline bool better(double k1, int i1, double k2, int i2) {
   return k1 > k2 || (k1 == k2 && i1 < i2);
@@ -100,8 +106,9 @@ __device__ inline void block_argmax(double& k, int& i, double* sk, int* si) {
 }
 
 // element j of a register row (j runtime, indices compile-time)
-__device__ __forceinline__ double rsel(const double (&r)[PW], int j) {
-  double x = 0.0;
+template <class S>
+__device__ __forceinline__ S rsel(const S (&r)[PW], int j) {
+  S x = S(0);
 #pragma unroll
   for (int c = 0; c < PW; c++)
     if (c == j) x = r[c];
@@ -125,6 +132,15 @@ __device__ inline int ld_agent(const int* p) {
 }
 __device__ inline void st_agent(int* p, int x) {
   __hip_atomic_store(p, x, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ inline float ld_agent(const float* p) {
+  const unsigned u = __hip_atomic_load(reinterpret_cast<unsigned*>(const_cast<float*>(p)),
+                                       __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  return __uint_as_float(u);
+}
+__device__ inline void st_agent(float* p, float x) {
+  __hip_atomic_store(reinterpret_cast<unsigned*>(p), __float_as_uint(x), __ATOMIC_RELAXED,
+                     __HIP_MEMORY_SCOPE_AGENT);
 }
 
 // ------------------------------------------------------ swap composition
@@ -213,9 +229,10 @@ laswp_compose(const int* __restrict__ ipiv, int k1, int k2, int inci, int* __res
 
 // Applies a composed list (at most 2*KC rows) to CW columns per workgroup:
 // gather the source rows into LDS, then scatter them to their destinations.
+template <class S>
 __global__ void __launch_bounds__(256)
-laswp_apply(long N, double* __restrict__ A, long lda, const int* __restrict__ lst) {
-  __shared__ double buf[2 * KC * CW];
+laswp_apply(long N, S* __restrict__ A, long lda, const int* __restrict__ lst) {
+  __shared__ S buf[2 * KC * CW];
   const int T = lst[0];
   const long c0 = (long)blockIdx.x * CW;
   const int nc = (int)min((long)CW, N - c0);
@@ -233,16 +250,18 @@ laswp_apply(long N, double* __restrict__ A, long lda, const int* __restrict__ ls
 // ---------------------------------------------------------------- panels
 
 // Global scratch of one cooperative panel launch.
+template <class S>
 struct CoopBuf {
   unsigned bar;  // barrier arrivals (reset to 0 before each launch)
   int err;       // 1 when a barrier wait ran past its bound
   int pad[2];
   Cand cand[2][CP_MAXG];           // per column parity: workgroup candidates
-  double crow[2][CP_MAXG][PW];     // ... and their candidate rows
-  double jrow[2][PW];              // ... and row j
+  S crow[2][CP_MAXG][PW];          // ... and their candidate rows
+  S jrow[2][PW];                   // ... and row j
 };
 
-__global__ void coop_reset(CoopBuf* cb, int* list) {
+template <class S>
+__global__ void coop_reset(CoopBuf<S>* cb, int* list) {
   cb->bar = 0u;
   cb->err = 0;
   list[0] = 0;
@@ -257,7 +276,8 @@ __global__ void coop_reset(CoopBuf* cb, int* list) {
 // broadcast through LDS so that every thread of the workgroup takes the same
 // branch.  Returns false when the wait ran past its bound (in any
 // workgroup).
-__device__ bool grid_sync(CoopBuf* cb, unsigned target, int* verdict) {
+template <class S>
+__device__ bool grid_sync(CoopBuf<S>* cb, unsigned target, int* verdict) {
   __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
   __syncthreads();
   if (threadIdx.x == 0) {
@@ -281,24 +301,24 @@ __device__ bool grid_sync(CoopBuf* cb, unsigned target, int* verdict) {
 // the matrix, on the diagonal).  Row i of the panel is held by global thread
 // i mod (G*256), RPT rows per thread.  Writes ipiv[off + j] (absolute),
 // *dinfo (first zero pivot), and the composed interchange list.
-template <int RPT>
+template <class S, int RPT>
 __global__ void __launch_bounds__(CP_THREADS)
-panel_coop(int M, int w, int off, double* __restrict__ P, long lda, int* __restrict__ ipiv,
-           int* __restrict__ dinfo, CoopBuf* __restrict__ cb, int* __restrict__ list) {
-  __shared__ double prow[PW], jrow[PW];
-  __shared__ double crows[CP_MAXG * PW];
+panel_coop(int M, int w, int off, S* __restrict__ P, long lda, int* __restrict__ ipiv,
+           int* __restrict__ dinfo, CoopBuf<S>* __restrict__ cb, int* __restrict__ list) {
+  __shared__ S prow[PW], jrow[PW];
+  __shared__ S crows[CP_MAXG * PW];
   __shared__ double sk[CP_THREADS / 64];
   __shared__ int si[CP_THREADS / 64];
   __shared__ int winner, verdict;
   __shared__ int pa[PW], pb[PW], cval[2 * PW], ccnt;
   const int t = threadIdx.x, G = gridDim.x, GT = G * CP_THREADS;
   const int g = blockIdx.x * CP_THREADS + t;
-  double v[RPT][PW];
+  S v[RPT][PW];
 #pragma unroll
   for (int q = 0; q < RPT; q++) {
     const int i = g + q * GT;
 #pragma unroll
-    for (int c = 0; c < PW; c++) v[q][c] = (i < M && c < w) ? P[i + (size_t)c * lda] : 0.0;
+    for (int c = 0; c < PW; c++) v[q][c] = (i < M && c < w) ? P[i + (size_t)c * lda] : S(0);
   }
   const int mn = min(M, w);
   bool ok = true;
@@ -349,9 +369,9 @@ panel_coop(int M, int w, int off, double* __restrict__ P, long lda, int* __restr
     // candidate rows and row j, then picks the winner locally
     for (int e = t; e < G * PW; e += CP_THREADS) {
       const int b = e / PW, c = e - b * PW;
-      crows[e] = (c < w) ? ld_agent(&cb->crow[par][b][c]) : 0.0;
+      crows[e] = (c < w) ? ld_agent(&cb->crow[par][b][c]) : S(0);
     }
-    if (t < PW) jrow[t] = (t < w) ? ld_agent(&cb->jrow[par][t]) : 0.0;
+    if (t < PW) jrow[t] = (t < w) ? ld_agent(&cb->jrow[par][t]) : S(0);
     if (t < 64) {
       double kk = -3.0;
       int ii = INT_MAX;
@@ -369,16 +389,16 @@ panel_coop(int M, int w, int off, double* __restrict__ P, long lda, int* __restr
     const int bw = (p % GT) / CP_THREADS;  // workgroup holding row p
     if (t < PW) prow[t] = crows[bw * PW + t];
     __syncthreads();
-    const double piv = prow[j];
+    const S piv = prow[j];
     if (blockIdx.x == 0 && t == 0) {
       ipiv[off + j] = off + p;
       pa[j] = off + j;
       pb[j] = off + p;
-      if (piv == 0.0 && *dinfo == 0) *dinfo = off + j + 1;
+      if (piv == S(0) && *dinfo == 0) *dinfo = off + j + 1;
     }
-    if (piv != 0.0) {
-      const bool recip = fabs(piv) >= DBL_MIN;
-      const double rinv = 1.0 / piv;
+    if (piv != S(0)) {
+      const bool recip = fabs(piv) >= std::numeric_limits<S>::min();
+      const S rinv = S(1) / piv;
 #pragma unroll
       for (int q = 0; q < RPT; q++) {
         const int i = g + q * GT;
@@ -392,7 +412,7 @@ panel_coop(int M, int w, int off, double* __restrict__ P, long lda, int* __restr
           }
         }
         if (i > j && i < M) {
-          double l = 0.0;
+          S l = S(0);
 #pragma unroll
           for (int c = 0; c < PW; c++)  // scale column j (fused select keeps v in VGPRs)
             if (c == j) {
@@ -426,8 +446,9 @@ panel_coop(int M, int w, int off, double* __restrict__ P, long lda, int* __restr
 }
 
 // Pivot record published by the last workgroup of a column step.
+template <class S>
 struct PivRec {
-  double piv;
+  S piv;
   int p;
   int pad;
 };
@@ -437,11 +458,12 @@ struct PivRec {
 // and rank-1 update to rows [j, M), then (j < mn) finds the pivot of column
 // j; the last workgroup to finish reduces the candidates, swaps rows j and
 // p across the panel and publishes the pivot (ipiv absolute: off + p).
+template <class S>
 __global__ void __launch_bounds__(STEP_THREADS)
-panel_step(int M, int w, int j, int mn, int off, double* __restrict__ A, long lda,
+panel_step(int M, int w, int j, int mn, int off, S* __restrict__ A, long lda,
            int* __restrict__ ipiv, int* __restrict__ dinfo, Cand* __restrict__ cand,
-           unsigned* __restrict__ ticket, PivRec* __restrict__ rec) {
-  __shared__ double urow[PW];
+           unsigned* __restrict__ ticket, PivRec<S>* __restrict__ rec) {
+  __shared__ S urow[PW];
   __shared__ double sk[4];
   __shared__ int si[4];
   __shared__ bool last;
@@ -449,13 +471,13 @@ panel_step(int M, int w, int j, int mn, int off, double* __restrict__ A, long ld
   const int r = j + blockIdx.x * STEP_THREADS + t;
   const bool mine = r < M;
   if (j > 0) {
-    const PivRec pr = rec[j - 1];
+    const PivRec<S> pr = rec[j - 1];
     if (t < w) urow[t] = A[(j - 1) + (size_t)t * lda];
     __syncthreads();
-    if (mine && pr.piv != 0.0) {
-      double* a = A + r;
-      double l = a[(size_t)(j - 1) * lda];
-      l = (fabs(pr.piv) >= DBL_MIN) ? l * (1.0 / pr.piv) : l / pr.piv;
+    if (mine && pr.piv != S(0)) {
+      S* a = A + r;
+      S l = a[(size_t)(j - 1) * lda];
+      l = (fabs(pr.piv) >= std::numeric_limits<S>::min()) ? l * (S(1) / pr.piv) : l / pr.piv;
       a[(size_t)(j - 1) * lda] = l;
       for (int c = j; c < w; c++) a[(size_t)c * lda] -= l * urow[c];
     }
@@ -487,16 +509,16 @@ panel_step(int M, int w, int j, int mn, int off, double* __restrict__ A, long ld
   }
   block_argmax(k2, i2, sk, si);
   const int p = i2;
-  const double piv = A[p + (size_t)j * lda];
-  if (piv != 0.0 && p != j && t < w) {
-    double* a = A + (size_t)t * lda;
-    const double x = a[j];
+  const S piv = A[p + (size_t)j * lda];
+  if (piv != S(0) && p != j && t < w) {
+    S* a = A + (size_t)t * lda;
+    const S x = a[j];
     a[j] = a[p];
     a[p] = x;
   }
   if (t == 0) {
     ipiv[off + j] = off + p;
-    if (piv == 0.0 && *dinfo == 0) *dinfo = off + j + 1;
+    if (piv == S(0) && *dinfo == 0) *dinfo = off + j + 1;
     rec[j].piv = piv;
     rec[j].p = p;
     *ticket = 0u;
@@ -507,67 +529,85 @@ __global__ void set_int_kernel(int* p, int v) { *p = v; }
 
 // ---------------------------------------------------------------- drivers
 
+template <class S>
 struct LU {
   hipStream_t s;
-  double* A;
+  S* A;
   long lda, M, N;
   int* ipiv;
   int* dinfo;
-  CoopBuf* cb;
+  CoopBuf<S>* cb;
   int* list;
   Cand* cand;        // panel_step scratch
   unsigned* ticket;
-  PivRec* rec;
-  double* Winv;      // inverses of the 128-blocks of L
+  PivRec<S>* rec;
+  S* Winv;           // inverses of the 128-blocks of L
   std::vector<char> inv_done;
+  int gemm_prec;     // kGemmNative or kGemmBf16 (float only)
 };
 
-double* at(const LU& f, long i, long j) { return f.A + i + j * f.lda; }
+// Trailing update C -= A * B of a recursion node.
+inline void node_update(LU<double>& f, long m, long n, long k, const double* A, double* B,
+                        double* C) {
+  gemm<double>(f.s, kNoTrans, kNoTrans, m, n, k, -1.0, A, f.lda, B, f.lda, 1.0, C, f.lda);
+}
+inline void node_update(LU<float>& f, long m, long n, long k, const float* A, float* B,
+                        float* C) {
+  if (f.gemm_prec == kGemmBf16)
+    gemm_bf16(f.s, m, n, k, -1.0f, A, f.lda, B, f.lda, 1.0f, C, f.lda);
+  else
+    gemm<float>(f.s, kNoTrans, kNoTrans, m, n, k, -1.0f, A, f.lda, B, f.lda, 1.0f, C, f.lda);
+}
+
+template <class S>
+S* at(const LU<S>& f, long i, long j) { return f.A + i + j * f.lda; }
 
 // Applies a composed list to all columns except [c0, c0 + w).
-void apply_outside(LU& f, long c0, long w, const int* lst) {
+template <class S>
+void apply_outside(LU<S>& f, long c0, long w, const int* lst) {
   if (c0 > 0) {
-    hipLaunchKernelGGL(laswp_apply, dim3((unsigned)((c0 + CW - 1) / CW)), dim3(256), 0, f.s, c0,
+    hipLaunchKernelGGL(laswp_apply<S>, dim3((unsigned)((c0 + CW - 1) / CW)), dim3(256), 0, f.s, c0,
                        f.A, f.lda, lst);
     ATLG_LAUNCHED();
   }
   const long r0 = c0 + w, nr = f.N - r0;
   if (nr > 0) {
-    hipLaunchKernelGGL(laswp_apply, dim3((unsigned)((nr + CW - 1) / CW)), dim3(256), 0, f.s, nr,
+    hipLaunchKernelGGL(laswp_apply<S>, dim3((unsigned)((nr + CW - 1) / CW)), dim3(256), 0, f.s, nr,
                        at(f, 0, r0), f.lda, lst);
     ATLG_LAUNCHED();
   }
 }
 
-template <int RPT>
-void launch_coop(LU& f, int M, int w, int off, int G) {
-  double* P = at(f, off, off);
+template <class S, int RPT>
+void launch_coop(LU<S>& f, int M, int w, int off, int G) {
+  S* P = at(f, off, off);
   long lda = f.lda;
   void* args[] = {&M, &w, &off, &P, &lda, &f.ipiv, &f.dinfo, &f.cb, &f.list};
-  ATLG_HIP(hipLaunchCooperativeKernel(reinterpret_cast<const void*>(&panel_coop<RPT>), dim3(G),
+  ATLG_HIP(hipLaunchCooperativeKernel(reinterpret_cast<const void*>(&panel_coop<S, RPT>), dim3(G),
                                       dim3(CP_THREADS), args, 0, f.s));
 }
 
 // Factors the leaf panel at (off, off): m rows, w columns, and applies its
 // interchanges to every other column.
-void leaf(LU& f, long off, long m, int w) {
+template <class S>
+void leaf(LU<S>& f, long off, long m, int w) {
   const long cap = (long)CP_MAXG * CP_THREADS;
   if (m <= 4 * cap) {
     const int rpt = m <= cap ? 1 : (m <= 2 * cap ? 2 : 4);
     const int G = (int)((m + (long)CP_THREADS * rpt - 1) / ((long)CP_THREADS * rpt));
-    hipLaunchKernelGGL(coop_reset, dim3(1), dim3(1), 0, f.s, f.cb, f.list);
+    hipLaunchKernelGGL(coop_reset<S>, dim3(1), dim3(1), 0, f.s, f.cb, f.list);
     ATLG_LAUNCHED();
-    if (rpt == 1) launch_coop<1>(f, (int)m, w, (int)off, G);
-    else if (rpt == 2) launch_coop<2>(f, (int)m, w, (int)off, G);
-    else launch_coop<4>(f, (int)m, w, (int)off, G);
+    if (rpt == 1) launch_coop<S, 1>(f, (int)m, w, (int)off, G);
+    else if (rpt == 2) launch_coop<S, 2>(f, (int)m, w, (int)off, G);
+    else launch_coop<S, 4>(f, (int)m, w, (int)off, G);
     apply_outside(f, off, w, f.list);
     return;
   }
   const int mn = (int)std::min<long>(m, w);
-  double* P = at(f, off, off);
+  S* P = at(f, off, off);
   for (int j = 0; j <= mn; j++) {
     const unsigned nb = (unsigned)std::max<long>(1, (m - j + STEP_THREADS - 1) / STEP_THREADS);
-    hipLaunchKernelGGL(panel_step, dim3(nb), dim3(STEP_THREADS), 0, f.s, (int)m, w, j, mn,
+    hipLaunchKernelGGL(panel_step<S>, dim3(nb), dim3(STEP_THREADS), 0, f.s, (int)m, w, j, mn,
                        (int)off, P, f.lda, f.ipiv, f.dinfo, f.cand, f.ticket, f.rec);
     ATLG_LAUNCHED();
   }
@@ -578,27 +618,29 @@ void leaf(LU& f, long off, long m, int w) {
 }
 
 // U12 <- L11^-1 A12 for the node at `off` with nl pivot columns.
-void node_trsm(LU& f, long off, long nl, long nr) {
+template <class S>
+void node_trsm(LU<S>& f, long off, long nl, long nr) {
   if (nl >= kTrsmNB && off % kTrsmNB == 0 && nl % kTrsmNB == 0) {
     const long b0 = off / kTrsmNB, b1 = (off + nl) / kTrsmNB;
     long first = b0;
     while (first < b1 && f.inv_done[first]) first++;
     if (first < b1) {
-      tri_inverse_blocks(f.s, (b1 - first) * kTrsmNB, at(f, first * kTrsmNB, first * kTrsmNB),
+      tri_inverse_blocks<S>(f.s, (b1 - first) * kTrsmNB, at(f, first * kTrsmNB, first * kTrsmNB),
                          f.lda, kNoTrans, kLower, kUnit,
                          f.Winv + (size_t)first * kTrsmNB * kTrsmNB);
       for (long b = first; b < b1; b++) f.inv_done[b] = 1;
     }
-    trsm(f.s, kLeft, kLower, kNoTrans, kUnit, nl, nr, 1.0, at(f, off, off), f.lda,
-         at(f, off, off + nl), f.lda, nullptr, f.Winv + (size_t)b0 * kTrsmNB * kTrsmNB, 0);
+    trsm<S>(f.s, kLeft, kLower, kNoTrans, kUnit, nl, nr, S(1), at(f, off, off), f.lda,
+            at(f, off, off + nl), f.lda, nullptr, f.Winv + (size_t)b0 * kTrsmNB * kTrsmNB, 0);
   } else {
-    trsm(f.s, kLeft, kLower, kNoTrans, kUnit, nl, nr, 1.0, at(f, off, off), f.lda,
-         at(f, off, off + nl), f.lda);
+    trsm<S>(f.s, kLeft, kLower, kNoTrans, kUnit, nl, nr, S(1), at(f, off, off), f.lda,
+            at(f, off, off + nl), f.lda);
   }
 }
 
 // Factors rows [off, M) x columns [off, off + n).
-void getrf_rec(LU& f, long off, long n) {
+template <class S>
+void getrf_rec(LU<S>& f, long off, long n) {
   const long m = f.M - off, mn = std::min(m, n);
   if (mn <= 0) return;
   if (n <= PW) {
@@ -616,8 +658,8 @@ void getrf_rec(LU& f, long off, long n) {
   const long nr = n - nl;
   getrf_rec(f, off, nl);
   node_trsm(f, off, nl, nr);
-  gemm(f.s, kNoTrans, kNoTrans, m - nl, nr, nl, -1.0, at(f, off + nl, off), f.lda,
-       at(f, off, off + nl), f.lda, 1.0, at(f, off + nl, off + nl), f.lda);
+  node_update(f, m - nl, nr, nl, at(f, off + nl, off), at(f, off, off + nl),
+              at(f, off + nl, off + nl));
   getrf_rec(f, off + nl, nr);
 }
 
@@ -628,8 +670,8 @@ void set_int(hipStream_t s, int* p, int v) {
   ATLG_LAUNCHED();
 }
 
-void laswp(hipStream_t s, long N, double* A, long lda, int K1, int K2, const int* ipiv,
-           int inci) {
+template <class S>
+void laswp(hipStream_t s, long N, S* A, long lda, int K1, int K2, const int* ipiv, int inci) {
   if (N <= 0 || K2 <= K1) return;
   int* lst = static_cast<int*>(scratch(s, kSlotLaswp, sizeof(int) * (1 + 4 * KC)));
   const unsigned nb = (unsigned)((N + CW - 1) / CW);
@@ -645,15 +687,17 @@ void laswp(hipStream_t s, long N, double* A, long lda, int K1, int K2, const int
     }
     hipLaunchKernelGGL(laswp_compose, dim3(1), dim3(KC), 0, s, ipiv, a, b, inci, lst);
     ATLG_LAUNCHED();
-    hipLaunchKernelGGL(laswp_apply, dim3(nb), dim3(256), 0, s, N, A, lda, lst);
+    hipLaunchKernelGGL(laswp_apply<S>, dim3(nb), dim3(256), 0, s, N, A, lda, lst);
     ATLG_LAUNCHED();
   }
 }
 
-void getrf(hipStream_t s, long M, long N, double* A, long lda, int* ipiv, int* dinfo) {
+template <class S>
+void getrf(hipStream_t s, long M, long N, S* A, long lda, int* ipiv, int* dinfo, int gemm_prec) {
   set_int(s, dinfo, 0);
   if (M == 0 || N == 0) return;
-  LU f;
+  LU<S> f;
+  f.gemm_prec = gemm_prec;
   f.s = s;
   f.A = A;
   f.lda = lda;
@@ -662,36 +706,46 @@ void getrf(hipStream_t s, long M, long N, double* A, long lda, int* ipiv, int* d
   f.ipiv = ipiv;
   f.dinfo = dinfo;
   const long maxblk = (M + STEP_THREADS - 1) / STEP_THREADS + 1;
-  const size_t b_cb = (sizeof(CoopBuf) + 255) & ~size_t(255);
+  const size_t b_cb = (sizeof(CoopBuf<S>) + 255) & ~size_t(255);
   const size_t b_list = 256 + sizeof(int) * (1 + 4 * KC);
-  const size_t b_rec = 256 + sizeof(PivRec) * PW;
+  const size_t b_rec = 256 + sizeof(PivRec<S>) * PW;
   const size_t b_cand = sizeof(Cand) * maxblk;
   char* base = static_cast<char*>(scratch(s, kSlotPanel, b_cb + b_list + 256 + b_rec + b_cand));
-  f.cb = reinterpret_cast<CoopBuf*>(base);
+  f.cb = reinterpret_cast<CoopBuf<S>*>(base);
   f.list = reinterpret_cast<int*>(base + b_cb);
   f.ticket = reinterpret_cast<unsigned*>(base + b_cb + b_list);
-  f.rec = reinterpret_cast<PivRec*>(base + b_cb + b_list + 256);
+  f.rec = reinterpret_cast<PivRec<S>*>(base + b_cb + b_list + 256);
   f.cand = reinterpret_cast<Cand*>(base + b_cb + b_list + 256 + b_rec);
   set_int(s, reinterpret_cast<int*>(f.ticket), 0);
   const long nblk = (std::min(M, N) + kTrsmNB - 1) / kTrsmNB;
-  f.Winv = static_cast<double*>(
-      scratch(s, kSlotFactInv, sizeof(double) * std::max<long>(nblk, 1) * kTrsmNB * kTrsmNB));
+  f.Winv = static_cast<S*>(
+      scratch(s, kSlotFactInv, sizeof(S) * std::max<long>(nblk, 1) * kTrsmNB * kTrsmNB));
   f.inv_done.assign(nblk + 1, 0);
   getrf_rec(f, 0, N);
 }
 
-void getrs(hipStream_t s, int trans, long N, long NRHS, const double* A, long lda,
-           const int* ipiv, double* B, long ldb) {
+template <class S>
+void getrs(hipStream_t s, int trans, long N, long NRHS, const S* A, long lda, const int* ipiv,
+           S* B, long ldb) {
   if (N == 0 || NRHS == 0) return;
   if (!is_trans(trans)) {
-    laswp(s, NRHS, B, ldb, 0, (int)N, ipiv, 1);
-    trsm(s, kLeft, kLower, kNoTrans, kUnit, N, NRHS, 1.0, A, lda, B, ldb);
-    trsm(s, kLeft, kUpper, kNoTrans, kNonUnit, N, NRHS, 1.0, A, lda, B, ldb);
+    laswp<S>(s, NRHS, B, ldb, 0, (int)N, ipiv, 1);
+    trsm<S>(s, kLeft, kLower, kNoTrans, kUnit, N, NRHS, S(1), A, lda, B, ldb);
+    trsm<S>(s, kLeft, kUpper, kNoTrans, kNonUnit, N, NRHS, S(1), A, lda, B, ldb);
   } else {
-    trsm(s, kLeft, kUpper, kTrans, kNonUnit, N, NRHS, 1.0, A, lda, B, ldb);
-    trsm(s, kLeft, kLower, kTrans, kUnit, N, NRHS, 1.0, A, lda, B, ldb);
-    laswp(s, NRHS, B, ldb, 0, (int)N, ipiv, -1);
+    trsm<S>(s, kLeft, kUpper, kTrans, kNonUnit, N, NRHS, S(1), A, lda, B, ldb);
+    trsm<S>(s, kLeft, kLower, kTrans, kUnit, N, NRHS, S(1), A, lda, B, ldb);
+    laswp<S>(s, NRHS, B, ldb, 0, (int)N, ipiv, -1);
   }
 }
+
+template void laswp<double>(hipStream_t, long, double*, long, int, int, const int*, int);
+template void laswp<float>(hipStream_t, long, float*, long, int, int, const int*, int);
+template void getrf<double>(hipStream_t, long, long, double*, long, int*, int*, int);
+template void getrf<float>(hipStream_t, long, long, float*, long, int*, int*, int);
+template void getrs<double>(hipStream_t, int, long, long, const double*, long, const int*,
+                            double*, long);
+template void getrs<float>(hipStream_t, int, long, long, const float*, long, const int*, float*,
+                           long);
 
 }  // namespace atlg

@@ -212,4 +212,219 @@ int atlg_dpotrs(int Uplo, long N, long NRHS, const double* A, long lda,
   return guarded([&] { potrs(resolve_stream(stream), Uplo, N, NRHS, A, lda, B, ldb); });
 }
 
+int atlg_dpotrf_adjoint(long N, const double* L, long ldl, double* G, long ldg,
+                        atlg_stream_t stream) {
+  if (N < 0) return -1;
+  if (ldl < std::max(1L, N)) return -3;
+  if (ldg < std::max(1L, N)) return -5;
+  return guarded([&] { potrf_adjoint(resolve_stream(stream), N, L, ldl, G, ldg); });
+}
+
+int atlg_dtrsm_adjoint(long N, long NRHS, const double* L, long ldl, const double* X, long ldx,
+                       double* Xbar, long ldxb, double* Lbar, long ldlb, atlg_stream_t stream) {
+  if (N < 0) return -1;
+  if (NRHS < 0) return -2;
+  if (ldl < std::max(1L, N)) return -4;
+  if (ldx < std::max(1L, N)) return -6;
+  if (ldxb < std::max(1L, N)) return -8;
+  if (Lbar && ldlb < std::max(1L, N)) return -10;
+  return guarded([&] {
+    trsm_adjoint(resolve_stream(stream), N, NRHS, L, ldl, X, ldx, Xbar, ldxb, Lbar, ldlb);
+  });
+}
+
+int atlg_dlogdet_chol(long N, const double* L, long ldl, double* out, atlg_stream_t stream) {
+  if (N < 0) return -1;
+  if (ldl < std::max(1L, N)) return -3;
+  return guarded([&] { logdet_chol(resolve_stream(stream), N, L, ldl, out); });
+}
+
+int atlg_dlogdet_chol_adjoint(long N, const double* L, long ldl, const double* dbar,
+                              double* Lbar, long ldlb, atlg_stream_t stream) {
+  if (N < 0) return -1;
+  if (ldl < std::max(1L, N)) return -3;
+  if (ldlb < std::max(1L, N)) return -6;
+  return guarded([&] { logdet_chol_adjoint(resolve_stream(stream), N, L, ldl, dbar, Lbar, ldlb); });
+}
+
+
+int atlg_sgemm(int TA, int TB, long M, long N, long K, float alpha, const float* A, long lda,
+               const float* B, long ldb, float beta, float* C, long ldc, atlg_stream_t stream) {
+  if (bad_trans(TA)) return -1;
+  if (bad_trans(TB)) return -2;
+  if (M < 0) return -3;
+  if (N < 0) return -4;
+  if (K < 0) return -5;
+  if (lda < std::max(1L, is_trans(TA) ? K : M)) return -8;
+  if (ldb < std::max(1L, is_trans(TB) ? N : K)) return -10;
+  if (ldc < std::max(1L, M)) return -13;
+  return guarded([&] {
+    gemm<float>(resolve_stream(stream), TA, TB, M, N, K, alpha, A, lda, B, ldb, beta, C, ldc);
+  });
+}
+
+int atlg_sgemm_bf16(long M, long N, long K, float alpha, const float* A, long lda,
+                    const float* B, long ldb, float beta, float* C, long ldc,
+                    atlg_stream_t stream) {
+  if (M < 0) return -1;
+  if (N < 0) return -2;
+  if (K < 0) return -3;
+  if (lda < std::max(1L, M)) return -6;
+  if (ldb < std::max(1L, K)) return -8;
+  if (ldc < std::max(1L, M)) return -11;
+  return guarded([&] {
+    gemm_bf16(resolve_stream(stream), M, N, K, alpha, A, lda, B, ldb, beta, C, ldc);
+  });
+}
+
+int atlg_sgetrf(long M, long N, float* A, long lda, int* ipiv, int* dinfo, int gemm_prec,
+                atlg_stream_t stream) {
+  if (M < 0) return -1;
+  if (N < 0) return -2;
+  if (lda < std::max(1L, M)) return -4;
+  if (!dinfo) return -6;
+  if (gemm_prec != kGemmNative && gemm_prec != kGemmBf16) return -7;
+  return guarded(
+      [&] { getrf<float>(resolve_stream(stream), M, N, A, lda, ipiv, dinfo, gemm_prec); });
+}
+
+int atlg_sgetrs(int Trans, long N, long NRHS, const float* A, long lda, const int* ipiv,
+                float* B, long ldb, atlg_stream_t stream) {
+  if (bad_trans(Trans)) return -1;
+  if (N < 0) return -2;
+  if (NRHS < 0) return -3;
+  if (lda < std::max(1L, N)) return -5;
+  if (ldb < std::max(1L, N)) return -8;
+  return guarded(
+      [&] { getrs<float>(resolve_stream(stream), Trans, N, NRHS, A, lda, ipiv, B, ldb); });
+}
+
+int atlg_dsgesv(long N, long NRHS, double* A, long lda, int* ipiv, const double* B, long ldb,
+                double* X, long ldx, int* dinfo, int itermax, int lowprec, int* iter,
+                atlg_stream_t stream) {
+  if (N < 0) return -1;
+  if (NRHS < 0) return -2;
+  if (lda < std::max(1L, N)) return -4;
+  if (ldb < std::max(1L, N)) return -7;
+  if (ldx < std::max(1L, N)) return -9;
+  if (!dinfo) return -10;
+  if (!iter) return -13;
+  return guarded([&] {
+    dsgesv(resolve_stream(stream), N, NRHS, A, lda, ipiv, B, ldb, X, ldx, dinfo,
+           itermax > 0 ? itermax : 30, lowprec ? 1 : 0, iter);
+  });
+}
+
+namespace {
+// Argument check shared by the batched entry points; returns 0 or -pos.
+int check_batch(int nsys, const int* n, const void* A, const int* lda, int pos_n, int pos_A,
+                int pos_lda) {
+  if (nsys < 0) return -1;
+  if (nsys == 0) return 0;
+  if (!n) return -pos_n;
+  if (!A) return -pos_A;
+  if (!lda) return -pos_lda;
+  for (int k = 0; k < nsys; k++) {
+    if (n[k] < 0) return -pos_n;
+    if (lda[k] < std::max(1, n[k])) return -pos_lda;
+  }
+  return 0;
+}
+int check_rhs(int nsys, const int* n, const void* B, const int* ldb, int pos_B, int pos_ldb) {
+  if (nsys == 0) return 0;
+  if (!B) return -pos_B;
+  if (!ldb) return -pos_ldb;
+  for (int k = 0; k < nsys; k++)
+    if (ldb[k] < std::max(1, n[k])) return -pos_ldb;
+  return 0;
+}
+}  // namespace
+
+int atlg_dpotrf_batched(int nsys, const int* n, double* const* A, const int* lda, int* info,
+                        atlg_stream_t stream) {
+  if (int e = check_batch(nsys, n, A, lda, 2, 3, 4)) return e;
+  if (nsys > 0 && !info) return -5;
+  return guarded([&] {
+    posv_batched<double>(resolve_stream(stream), 0, nsys, n, A, lda, nullptr, nullptr, nullptr,
+                         0, info, batch_tune(false));
+  });
+}
+
+int atlg_dpotrs_batched(int nsys, const int* n, double* const* A, const int* lda,
+                        double* const* B, const int* ldb, int nrhs, atlg_stream_t stream) {
+  if (int e = check_batch(nsys, n, A, lda, 2, 3, 4)) return e;
+  if (int e = check_rhs(nsys, n, B, ldb, 5, 6)) return e;
+  if (nrhs < 0) return -7;
+  if (nrhs == 0) return 0;
+  return guarded([&] {
+    posv_batched<double>(resolve_stream(stream), 2, nsys, n, A, lda, B, ldb, nullptr, nrhs,
+                         nullptr, batch_tune(false));
+  });
+}
+
+int atlg_dposv_batched(int nsys, const int* n, double* const* A, const int* lda,
+                       double* const* B, const int* ldb, int nrhs, int* info,
+                       atlg_stream_t stream) {
+  if (int e = check_batch(nsys, n, A, lda, 2, 3, 4)) return e;
+  if (int e = check_rhs(nsys, n, B, ldb, 5, 6)) return e;
+  if (nrhs < 0) return -7;
+  if (nsys > 0 && !info) return -8;
+  return guarded([&] {
+    posv_batched<double>(resolve_stream(stream), nrhs ? 1 : 0, nsys, n, A, lda, B, ldb, nullptr,
+                         nrhs, info, batch_tune(false));
+  });
+}
+
+int atlg_spotrf_batched(int nsys, const int* n, float* const* A, const int* lda, int* info,
+                        atlg_stream_t stream) {
+  if (int e = check_batch(nsys, n, A, lda, 2, 3, 4)) return e;
+  if (nsys > 0 && !info) return -5;
+  return guarded([&] {
+    posv_batched<float>(resolve_stream(stream), 0, nsys, n, A, lda, nullptr, nullptr, nullptr, 0,
+                        info, batch_tune(true));
+  });
+}
+
+int atlg_sposv_batched(int nsys, const int* n, float* const* A, const int* lda, float* const* B,
+                       const int* ldb, int nrhs, int* info, atlg_stream_t stream) {
+  if (int e = check_batch(nsys, n, A, lda, 2, 3, 4)) return e;
+  if (int e = check_rhs(nsys, n, B, ldb, 5, 6)) return e;
+  if (nrhs < 0) return -7;
+  if (nsys > 0 && !info) return -8;
+  return guarded([&] {
+    posv_batched<float>(resolve_stream(stream), nrhs ? 1 : 0, nsys, n, A, lda, B, ldb, nullptr,
+                        nrhs, info, batch_tune(true));
+  });
+}
+
+int atlg_dsposv_batched(int nsys, const int* n, double* const* A, const int* lda,
+                        const double* const* B, const int* ldb, double* const* X, const int* ldx,
+                        int nrhs, int itermax, int* info, int* iter, atlg_stream_t stream) {
+  if (int e = check_batch(nsys, n, A, lda, 2, 3, 4)) return e;
+  if (int e = check_rhs(nsys, n, B, ldb, 5, 6)) return e;
+  if (int e = check_rhs(nsys, n, X, ldx, 7, 8)) return e;
+  if (nrhs < 1) return -9;
+  if (nsys > 0 && !info) return -11;
+  if (nsys > 0 && !iter) return -12;
+  return guarded([&] {
+    dsposv_batched(resolve_stream(stream), nsys, n, A, lda, B, ldb, X, ldx, nrhs, info, iter,
+                   itermax > 0 ? itermax : 30, batch_tune(true));
+  });
+}
+
+int atlg_batched_tune_get(int prec, int* t16_max, int* wg_per_cu) {
+  if (prec != 0 && prec != 1) return -1;
+  const BatchTune t = batch_tune(prec == 1);
+  if (t16_max) *t16_max = t.t16_max;
+  if (wg_per_cu) *wg_per_cu = t.wg_per_cu;
+  return 0;
+}
+
+int atlg_batched_tune_set(int prec, int t16_max, int wg_per_cu) {
+  if (prec != 0 && prec != 1) return -1;
+  if (wg_per_cu < 1 || wg_per_cu > 16) return -3;
+  set_batch_tune(prec == 1, BatchTune{t16_max, wg_per_cu});
+  return 0;
+}
+
 }  // extern "C"

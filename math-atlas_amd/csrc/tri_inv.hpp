@@ -7,7 +7,7 @@
 namespace atlg {
 
 constexpr int kTriN = 128;          // block order
-constexpr int kTriLD = kTriN + 1;   // LDS pitch (doubles)
+constexpr int kTriLD = kTriN + 1;   // LDS pitch (elements)
 constexpr int kTriThreads = 256;
 
 // S[i*kTriLD + k] holds L(i,k) for i >= k and 0 above the diagonal; on return
@@ -16,27 +16,31 @@ constexpr int kTriThreads = 256;
 // thread (forward substitution with reciprocal diagonals), then for orders
 // 16, 32, 64, 128 each diagonal block [[X11, 0], [L21, X22]] with inverted
 // halves gets its off-diagonal part X21 = -X22 * (L21 * X11).
+// Only the leading order n (<= kTriN) is general: rows and columns at or
+// beyond n are the identity padding, so every off-diagonal block that starts
+// at or beyond n stays zero and is skipped.
 // Must be called by all kTriThreads threads; ends with a barrier.
-__device__ inline void lower_inverse_inplace(double* S) {
+template <class R>
+__device__ inline void lower_inverse_inplace(R* S, int n = kTriN) {
   const int t = threadIdx.x;
-  if (t < kTriN / 8) {
+  if (t < kTriN / 8 && 8 * t < n) {
     const int o = 8 * t;
-    double a[8][8], x[8][8];
+    R a[8][8], x[8][8];
 #pragma unroll
     for (int r = 0; r < 8; r++)
 #pragma unroll
-      for (int c = 0; c < 8; c++) a[r][c] = (c <= r) ? S[(o + r) * kTriLD + o + c] : 0.0;
+      for (int c = 0; c < 8; c++) a[r][c] = (c <= r) ? S[(o + r) * kTriLD + o + c] : R(0);
 #pragma unroll
     for (int c = 0; c < 8; c++) {
 #pragma unroll
-      for (int r = 0; r < 8; r++) x[r][c] = 0.0;
-      x[c][c] = 1.0 / a[c][c];
+      for (int r = 0; r < 8; r++) x[r][c] = R(0);
+      x[c][c] = R(1) / a[c][c];
 #pragma unroll
       for (int r = c + 1; r < 8; r++) {
-        double acc = 0.0;
+        R acc = R(0);
 #pragma unroll
         for (int k = c; k < r; k++) acc += a[r][k] * x[k][c];
-        x[r][c] = -acc * (1.0 / a[r][r]);
+        x[r][c] = -acc * (R(1) / a[r][r]);
       }
     }
 #pragma unroll
@@ -47,12 +51,13 @@ __device__ inline void lower_inverse_inplace(double* S) {
   __syncthreads();
   for (int sz = 16; sz <= kTriN; sz *= 2) {
     const int h = sz / 2, hh = h * h;
+    if (h >= n) break;  // the remaining levels only meet the identity padding
     const int cnt = (kTriN / sz) * hh / kTriThreads;  // 2, 4, 8, 16 per thread
-    double v[16];
+    R v[16];
     for (int u = 0; u < cnt; u++) {  // T = L21 * X11 (X11 lower)
       const int e = t + kTriThreads * u, q = e / hh, rem = e - q * hh;
       const int r = rem / h, c = rem - r * h, o = q * sz;
-      double acc = 0.0;
+      R acc = R(0);
       for (int k = c; k < h; k++) acc += S[(o + h + r) * kTriLD + o + k] * S[(o + k) * kTriLD + o + c];
       v[u] = acc;
     }
@@ -66,7 +71,7 @@ __device__ inline void lower_inverse_inplace(double* S) {
     for (int u = 0; u < cnt; u++) {  // X21 = -X22 * T (X22 lower)
       const int e = t + kTriThreads * u, q = e / hh, rem = e - q * hh;
       const int r = rem / h, c = rem - r * h, o = q * sz;
-      double acc = 0.0;
+      R acc = R(0);
       for (int k = 0; k <= r; k++)
         acc += S[(o + h + r) * kTriLD + o + h + k] * S[(o + h + k) * kTriLD + o + c];
       v[u] = -acc;

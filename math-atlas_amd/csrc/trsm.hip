@@ -1,4 +1,5 @@
-// Triangular solve with many right-hand sides on the FP64 matrix cores.
+// Triangular solve with many right-hand sides on the matrix cores (FP64, and
+// FP32 for the single-precision factorization of the mixed solver).
 //
 // Replaces ATL_dtrsm (AtlasBase/Clint/trsm-micro.base:200-420: recursive
 // halving trsm_recL/R, then iptrsm/trsmL_LNUT/utrsm with reciprocal
@@ -12,7 +13,7 @@
 //   2. a recursive halving driver (the same split as trsm_recL/R) applies
 //      each inverse to its block row/column as an in-place GEMM and updates
 //      the rest with large GEMMs.
-// All flops but the small inversions run on v_mfma_f64_16x16x4_f64.
+// All flops but the small inversions run on the 16x16x4 MFMA GEMM.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -28,7 +29,8 @@ constexpr int TRI_THREADS = kTriThreads;
 static_assert(NB == kTriN, "TRSM blocks are inverted by the 128-block inverter");
 
 // Element (i, j) of op(A).
-__device__ inline double opa(const double* A, long lda, bool trans, long i, long j) {
+template <class S>
+__device__ inline S opa(const S* A, long lda, bool trans, long i, long j) {
   return trans ? A[j + i * lda] : A[i + j * lda];
 }
 
@@ -37,58 +39,61 @@ __device__ inline double opa(const double* A, long lda, bool trans, long i, long
 // as the lower-triangular J U J (J = order reversal), so one in-place lower
 // inversion serves all cases.  Output: Dinv + b*NB*NB, column-major, ld NB,
 // zeros outside the triangle; Unit diagonals are never read.
+template <class R>
 __global__ void __launch_bounds__(TRI_THREADS)
-trtri_diag_kernel(int na, const double* __restrict__ A, long lda, int trans, int lower,
-                  int unit, double* __restrict__ Dinv, const int* __restrict__ skip) {
+trtri_diag_kernel(int na, const R* __restrict__ A, long lda, int trans, int lower, int unit,
+                  R* __restrict__ Dinv, const int* __restrict__ skip) {
   if (skip && *skip) return;
-  __shared__ double S[kTriN * kTriLD];
+  __shared__ R S[kTriN * kTriLD];
   const int b = blockIdx.x, t = threadIdx.x;
   const long o = (long)b * NB;
   const int n = (int)min((long)NB, na - o);
   const bool tr = trans != 0;
   for (int idx = t; idx < NB * NB; idx += TRI_THREADS) {
     const int i = idx % NB, j = idx / NB;
-    double v = 0.0;
+    R v = R(0);
     if (i >= j) {
       if (i < n && j < n) {
-        if (i == j && unit) v = 1.0;
+        if (i == j && unit) v = R(1);
         else v = lower ? opa(A, lda, tr, o + i, o + j)
                        : opa(A, lda, tr, o + n - 1 - i, o + n - 1 - j);
       } else if (i == j) {
-        v = 1.0;
+        v = R(1);
       }
     }
     S[i * kTriLD + j] = v;
   }
   __syncthreads();
-  lower_inverse_inplace(S);
-  double* D = Dinv + (size_t)b * NB * NB;
+  lower_inverse_inplace(S, n);
+  R* D = Dinv + (size_t)b * NB * NB;
   for (int idx = t; idx < NB * NB; idx += TRI_THREADS) {
     const int i = idx % NB, j = idx / NB;
-    double v = 0.0;
+    R v = R(0);
     if (i < n && j < n) {
-      if (lower) v = (i >= j) ? S[i * kTriLD + j] : 0.0;
-      else v = (i <= j) ? S[(n - 1 - i) * kTriLD + (n - 1 - j)] : 0.0;
+      if (lower) v = (i >= j) ? S[i * kTriLD + j] : R(0);
+      else v = (i <= j) ? S[(n - 1 - i) * kTriLD + (n - 1 - j)] : R(0);
     }
     D[i + (size_t)j * NB] = v;
   }
 }
 
+template <class S>
 struct Ctx {
   hipStream_t s;
   bool left, lower_eff, trans;
-  const double* A;
+  const S* A;
   long lda;
-  double* B;
+  S* B;
   long ldb;
   long M, N;   // B is M x N
-  const double* Dinv;
+  const S* Dinv;
   int dtrans;  // kTrans when Dinv holds the transposes of inv(op(A)_bb)
   const int* skip;
 };
 
 // op(A)[i0:, j0:] as a GEMM operand: pointer and transpose flag.
-inline void opa_block(const Ctx& c, long i0, long j0, const double** p, int* t) {
+template <class S>
+inline void opa_block(const Ctx<S>& c, long i0, long j0, const S** p, int* t) {
   if (!c.trans) {
     *p = c.A + i0 + j0 * c.lda;
     *t = kNoTrans;
@@ -104,84 +109,87 @@ long split(long n) {
 }
 
 // Solves on rows/cols [r0, r0+n) of op(A) with scale alpha.
-void solve(const Ctx& c, long r0, long n, double alpha) {
+template <class S>
+void solve(const Ctx<S>& c, long r0, long n, S alpha) {
+  const S one = S(1), mone = S(-1), zero = S(0);
   if (n <= NB) {
-    const double* D = c.Dinv + (size_t)(r0 / NB) * NB * NB;
+    const S* D = c.Dinv + (size_t)(r0 / NB) * NB * NB;
     if (c.left) {
-      double* Bb = c.B + r0;
-      gemm(c.s, c.dtrans, kNoTrans, n, c.N, n, alpha, D, NB, Bb, c.ldb, 0.0, Bb, c.ldb, 0,
-           c.skip);
+      S* Bb = c.B + r0;
+      gemm<S>(c.s, c.dtrans, kNoTrans, n, c.N, n, alpha, D, NB, Bb, c.ldb, zero, Bb, c.ldb, 0,
+              c.skip);
     } else {
-      double* Bb = c.B + r0 * c.ldb;
-      gemm(c.s, kNoTrans, c.dtrans, c.M, n, n, alpha, Bb, c.ldb, D, NB, 0.0, Bb, c.ldb, 0,
-           c.skip);
+      S* Bb = c.B + r0 * c.ldb;
+      gemm<S>(c.s, kNoTrans, c.dtrans, c.M, n, n, alpha, Bb, c.ldb, D, NB, zero, Bb, c.ldb, 0,
+              c.skip);
     }
     return;
   }
   const long n1 = split(n), n2 = n - n1;
-  const double* Ap;
+  const S* Ap;
   int ta;
   if (c.left) {
-    double* B1 = c.B + r0;
-    double* B2 = c.B + r0 + n1;
+    S* B1 = c.B + r0;
+    S* B2 = c.B + r0 + n1;
     if (c.lower_eff) {
       // [L11 0; L21 L22]: X1 first, then B2 <- alpha*B2 - L21*X1
       solve(c, r0, n1, alpha);
       opa_block(c, r0 + n1, r0, &Ap, &ta);
-      gemm(c.s, ta, kNoTrans, n2, c.N, n1, -1.0, Ap, c.lda, B1, c.ldb, alpha, B2, c.ldb, 0,
+      gemm<S>(c.s, ta, kNoTrans, n2, c.N, n1, mone, Ap, c.lda, B1, c.ldb, alpha, B2, c.ldb, 0,
            c.skip);
-      solve(c, r0 + n1, n2, 1.0);
+      solve(c, r0 + n1, n2, one);
     } else {
       // [U11 U12; 0 U22]: X2 first, then B1 <- alpha*B1 - U12*X2
       solve(c, r0 + n1, n2, alpha);
       opa_block(c, r0, r0 + n1, &Ap, &ta);
-      gemm(c.s, ta, kNoTrans, n1, c.N, n2, -1.0, Ap, c.lda, B2, c.ldb, alpha, B1, c.ldb, 0,
+      gemm<S>(c.s, ta, kNoTrans, n1, c.N, n2, mone, Ap, c.lda, B2, c.ldb, alpha, B1, c.ldb, 0,
            c.skip);
-      solve(c, r0, n1, 1.0);
+      solve(c, r0, n1, one);
     }
   } else {
-    double* B1 = c.B + r0 * c.ldb;
-    double* B2 = c.B + (r0 + n1) * c.ldb;
+    S* B1 = c.B + r0 * c.ldb;
+    S* B2 = c.B + (r0 + n1) * c.ldb;
     if (c.lower_eff) {
       // X [L11 0; L21 L22] = B: X2 first, then B1 <- alpha*B1 - X2*L21
       solve(c, r0 + n1, n2, alpha);
       opa_block(c, r0 + n1, r0, &Ap, &ta);
-      gemm(c.s, kNoTrans, ta, c.M, n1, n2, -1.0, B2, c.ldb, Ap, c.lda, alpha, B1, c.ldb, 0,
+      gemm<S>(c.s, kNoTrans, ta, c.M, n1, n2, mone, B2, c.ldb, Ap, c.lda, alpha, B1, c.ldb, 0,
            c.skip);
-      solve(c, r0, n1, 1.0);
+      solve(c, r0, n1, one);
     } else {
       // X [U11 U12; 0 U22] = B: X1 first, then B2 <- alpha*B2 - X1*U12
       solve(c, r0, n1, alpha);
       opa_block(c, r0, r0 + n1, &Ap, &ta);
-      gemm(c.s, kNoTrans, ta, c.M, n2, n1, -1.0, B1, c.ldb, Ap, c.lda, alpha, B2, c.ldb, 0,
+      gemm<S>(c.s, kNoTrans, ta, c.M, n2, n1, mone, B1, c.ldb, Ap, c.lda, alpha, B2, c.ldb, 0,
            c.skip);
-      solve(c, r0 + n1, n2, 1.0);
+      solve(c, r0 + n1, n2, one);
     }
   }
 }
 
 }  // namespace
 
-void tri_inverse_blocks(hipStream_t s, long na, const double* A, long lda, int ta, int uplo,
-                        int diag, double* Dinv, const int* skip) {
+template <class S>
+void tri_inverse_blocks(hipStream_t s, long na, const S* A, long lda, int ta, int uplo, int diag,
+                        S* Dinv, const int* skip) {
   if (na <= 0) return;
   const bool tr = is_trans(ta);
   const long nblk = (na + NB - 1) / NB;
-  hipLaunchKernelGGL(trtri_diag_kernel, dim3((unsigned)nblk), dim3(TRI_THREADS), 0, s, (int)na,
+  hipLaunchKernelGGL(trtri_diag_kernel<S>, dim3((unsigned)nblk), dim3(TRI_THREADS), 0, s, (int)na,
                      A, lda, tr ? 1 : 0, (uplo == kLower) != tr ? 1 : 0, diag == kUnit ? 1 : 0,
                      Dinv, skip);
   ATLG_LAUNCHED();
 }
 
-void trsm(hipStream_t s, int side, int uplo, int ta, int diag, long M, long N, double alpha,
-          const double* A, long lda, double* B, long ldb, const int* skip, const double* dinv,
-          int dinv_trans) {
+template <class S>
+void trsm(hipStream_t s, int side, int uplo, int ta, int diag, long M, long N, S alpha,
+          const S* A, long lda, S* B, long ldb, const int* skip, const S* dinv, int dinv_trans) {
   if (M <= 0 || N <= 0) return;
-  if (alpha == 0.0) {
-    scale(s, M, N, 0.0, B, ldb, 0, skip);
+  if (alpha == S(0)) {
+    scale<S>(s, M, N, S(0), B, ldb, 0, skip);
     return;
   }
-  Ctx c;
+  Ctx<S> c;
   c.s = s;
   c.left = (side == kLeft);
   c.trans = is_trans(ta);
@@ -199,11 +207,20 @@ void trsm(hipStream_t s, int side, int uplo, int ta, int diag, long M, long N, d
     c.Dinv = dinv;
   } else {
     const long nblk = (na + NB - 1) / NB;
-    double* D = static_cast<double*>(scratch(s, kSlotTrsmInv, sizeof(double) * nblk * NB * NB));
-    tri_inverse_blocks(s, na, A, lda, ta, uplo, diag, D, skip);
+    S* D = static_cast<S*>(scratch(s, kSlotTrsmInv, sizeof(S) * nblk * NB * NB));
+    tri_inverse_blocks<S>(s, na, A, lda, ta, uplo, diag, D, skip);
     c.Dinv = D;
   }
   solve(c, 0, na, alpha);
 }
+
+template void tri_inverse_blocks<double>(hipStream_t, long, const double*, long, int, int, int,
+                                         double*, const int*);
+template void tri_inverse_blocks<float>(hipStream_t, long, const float*, long, int, int, int,
+                                        float*, const int*);
+template void trsm<double>(hipStream_t, int, int, int, int, long, long, double, const double*,
+                           long, double*, long, const int*, const double*, int);
+template void trsm<float>(hipStream_t, int, int, int, int, long, long, float, const float*, long,
+                          float*, long, const int*, const float*, int);
 
 }  // namespace atlg

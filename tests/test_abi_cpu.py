@@ -94,3 +94,37 @@ def test_device_api_rejects_bad_arguments_without_gpu(G):
     assert L.atlg_dtrsm(G.Left, G.Lower, G.NoTrans, G.Unit, 4, 4, 1.0, nul, 4, nul, 3, nul) == -11
     assert L.atlg_dgetrf(4, 4, nul, 4, nul, nul, nul) == -6
     assert L.atlg_dpotrf(G.Lower, 4, nul, 2, nul, nul) == -4
+
+
+def test_batched_argument_errors(G):
+    """The batched entry points check every system's order and leading
+    dimensions on the host before any device work."""
+    L = G.lib()
+    c = ctypes
+    n = (c.c_int * 2)(4, -1)
+    lda = (c.c_int * 2)(4, 4)
+    ptrs = (c.c_void_p * 2)(16, 16)
+    info = c.c_void_p(16)
+    s = c.c_void_p(0)
+    assert L.atlg_dpotrf_batched(-1, n, ptrs, lda, info, s) == -1
+    assert L.atlg_dpotrf_batched(2, n, ptrs, lda, info, s) == -2
+    n[1] = 5
+    assert L.atlg_dpotrf_batched(2, n, ptrs, lda, info, s) == -4
+    assert L.atlg_dpotrf_batched(2, n, ptrs, lda, None, s) == -4
+    lda[1] = 5
+    assert L.atlg_dpotrf_batched(2, n, ptrs, lda, None, s) == -5
+    ldb = (c.c_int * 2)(4, 4)
+    assert L.atlg_dposv_batched(2, n, ptrs, lda, ptrs, ldb, 4, info, s) == -6
+    ldb[1] = 5
+    assert L.atlg_dposv_batched(2, n, ptrs, lda, ptrs, ldb, -1, info, s) == -7
+    assert L.atlg_dposv_batched(2, n, ptrs, lda, None, ldb, 4, info, s) == -5
+    assert L.atlg_dsposv_batched(2, n, ptrs, lda, ptrs, ldb, ptrs, ldb, 0, 30, info, info,
+                                 s) == -9
+    assert L.atlg_dsposv_batched(2, n, ptrs, lda, ptrs, ldb, ptrs, ldb, 4, 30, info, None,
+                                 s) == -12
+    assert L.atlg_dpotrf_batched(0, n, ptrs, lda, None, s) == 0
+    assert L.atlg_batched_tune_set(2, 64, 2) == -1
+    assert L.atlg_batched_tune_set(0, 64, 0) == -3
+    t, w = c.c_int(0), c.c_int(0)
+    assert L.atlg_batched_tune_get(0, c.byref(t), c.byref(w)) == 0
+    assert w.value >= 1
