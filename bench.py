@@ -107,6 +107,27 @@ def sweep(torch, G, only="", ir_n=32768):
         ms = timed(torch, run_trsm, 2) - ms_copy
         out["dtrsm_LLNU_16384x16384_tflops"] = float(N) * N * N / ms / 1e9
         del a, b, b0
+    if want("laswp"):
+        # ATL_dlaswp over all N columns with the interchanges of a partial-
+        # pivoting LU of a U[-1,1] matrix (taken from the DGETRF below on a
+        # smaller order would change the pattern; a seeded random pivot row
+        # per step in [i, N) has the same "row i receives row p" shape).
+        import numpy as np
+        rng = np.random.default_rng(60)
+        piv = np.array([rng.integers(i, N) for i in range(N)], dtype=np.int32)
+        ip = torch.from_numpy(piv).to("cuda")
+        a = uniform(torch, (N, N), 61)
+        ms = timed(torch, lambda: G.dlaswp(N, a, N, 0, N, ip, 1), 3)
+        # rows whose content changes, each read and written once per column
+        perm = np.arange(N)
+        for i in range(N):
+            p = piv[i]
+            perm[i], perm[p] = perm[p], perm[i]
+        moved = int((perm != np.arange(N)).sum())
+        out["laswp_16384x16384_ms"] = ms
+        out["laswp_16384x16384_rows_moved"] = moved
+        out["laswp_16384x16384_GBps"] = 2.0 * 8 * moved * N / ms / 1e6
+        del a
     if want("dgetrf"):
         a0 = uniform(torch, (N, N), 40)
         a = a0.clone()
@@ -144,12 +165,17 @@ def sweep(torch, G, only="", ir_n=32768):
         out["dpotrf_16384_ms"] = ms
         out["dpotrf_16384_info"] = int(info.item())
         del s, s0
-    if want("ir"):
-        out.update(bench_ir(torch, G, ir_n))
-    if want("batched"):
-        out.update(bench_batched(torch, G))
-    if want("cholgrad"):
-        out.update(bench_cholgrad(torch))
+    # the extensions report an error string instead of ending the run
+    for key, fn in (("ir", lambda: bench_ir(torch, G, ir_n)),
+                    ("batched", lambda: bench_batched(torch, G)),
+                    ("cholgrad", lambda: bench_cholgrad(torch))):
+        if not want(key):
+            continue
+        try:
+            out.update(fn())
+        except Exception as e:  # noqa: BLE001
+            out["%s_error" % key] = "%s: %s" % (type(e).__name__, e)
+        torch.cuda.empty_cache()
     torch.cuda.empty_cache()
     return out
 

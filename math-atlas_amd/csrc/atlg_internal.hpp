@@ -51,6 +51,12 @@ hipStream_t resolve_stream(atlg_stream_t s);
 // entry points (created on first use, default priority, blocking).
 hipStream_t library_stream();
 
+// A second library stream on the current device (non-blocking, default
+// priority) and two events, for work that overlaps the caller's stream
+// (getrf's look-ahead panel).  Created on first use.
+hipStream_t aux_stream();
+hipEvent_t aux_event(int i);
+
 // Per-stream scratch memory in named slots.  A slot grows (after a stream
 // synchronize) and is reused by later calls on the same stream; pointers from
 // different slots never alias.

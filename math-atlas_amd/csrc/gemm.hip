@@ -24,6 +24,14 @@ namespace atlg {
 namespace {
 
 constexpr int BM = 128, BN = 128, BK = 16, NT = 256;
+// Tile rows per group of the XCD-aware tile order, and whether workgroup ids
+// are regrouped per XCD at all (tuned constants).
+#ifndef ATLG_GEMM_GM
+#define ATLG_GEMM_GM 8
+#endif
+#ifndef ATLG_GEMM_XCD
+#define ATLG_GEMM_XCD 1
+#endif
 constexpr int WM = 64, WN = 64;           // wave tile
 constexpr int TM = WM / 16, TN = WN / 16; // MFMA tiles per wave
 constexpr int PADA = 4, PADB = 4;         // LDS row padding (elements)
@@ -135,9 +143,10 @@ __device__ inline void tile_of(int bid, int nblk, int tilesM, int tilesN, int& t
                                int& tn) {
   const int per = nblk / 8, rem = nblk % 8;
   const int xcd = bid % 8, local = bid / 8;
-  const int id = (xcd < rem) ? xcd * (per + 1) + local
-                             : rem * (per + 1) + (xcd - rem) * per + local;
-  constexpr int GM = 8;  // tile rows per group
+  const int id = !ATLG_GEMM_XCD ? bid
+                 : (xcd < rem) ? xcd * (per + 1) + local
+                               : rem * (per + 1) + (xcd - rem) * per + local;
+  constexpr int GM = ATLG_GEMM_GM;  // tile rows per group
   const int group = id / (GM * tilesN);
   const int first = group * GM;
   const int gsz = min(tilesM - first, GM);

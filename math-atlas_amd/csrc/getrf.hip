@@ -52,6 +52,18 @@ constexpr int STEP_THREADS = 256;
 constexpr int KC = 256;           // interchanges composed per laswp chunk
 constexpr int CW = 16;            // columns per laswp workgroup
 constexpr long kSpinLimit = 1L << 20;  // barrier polls (~0.1 s) before giving up
+// Fewest panel rows per thread of the cooperative panel (1, 2 or 4): more
+// rows per thread means fewer workgroups exchanging candidates per column.
+#ifndef ATLG_PANEL_RPT_MIN
+#define ATLG_PANEL_RPT_MIN 1
+#endif
+// Look-ahead driver (1) or ATLAS's recursion over the whole matrix (0), and
+// its block-column width and the smallest min(M, N) that uses it.
+#ifndef ATLG_LU_LOOKAHEAD
+#define ATLG_LU_LOOKAHEAD 0
+#endif
+constexpr long kLaNB = 256;
+constexpr long kLaMin = 2048;
 
 // ---------------------------------------------------------------- helpers
 
@@ -544,6 +556,7 @@ struct LU {
   S* Winv;           // inverses of the 128-blocks of L
   std::vector<char> inv_done;
   int gemm_prec;     // kGemmNative or kGemmBf16 (float only)
+  long win_lo, win_hi;  // leaves apply their interchanges to these columns only
 };
 
 // Trailing update C -= A * B of a recursion node.
@@ -562,15 +575,16 @@ inline void node_update(LU<float>& f, long m, long n, long k, const float* A, fl
 template <class S>
 S* at(const LU<S>& f, long i, long j) { return f.A + i + j * f.lda; }
 
-// Applies a composed list to all columns except [c0, c0 + w).
+// Applies a composed list to the window's columns except [c0, c0 + w).
 template <class S>
 void apply_outside(LU<S>& f, long c0, long w, const int* lst) {
-  if (c0 > 0) {
-    hipLaunchKernelGGL(laswp_apply<S>, dim3((unsigned)((c0 + CW - 1) / CW)), dim3(256), 0, f.s, c0,
-                       f.A, f.lda, lst);
+  const long nl = c0 - f.win_lo;
+  if (nl > 0) {
+    hipLaunchKernelGGL(laswp_apply<S>, dim3((unsigned)((nl + CW - 1) / CW)), dim3(256), 0, f.s, nl,
+                       at(f, 0, f.win_lo), f.lda, lst);
     ATLG_LAUNCHED();
   }
-  const long r0 = c0 + w, nr = f.N - r0;
+  const long r0 = c0 + w, nr = f.win_hi - r0;
   if (nr > 0) {
     hipLaunchKernelGGL(laswp_apply<S>, dim3((unsigned)((nr + CW - 1) / CW)), dim3(256), 0, f.s, nr,
                        at(f, 0, r0), f.lda, lst);
@@ -593,7 +607,7 @@ template <class S>
 void leaf(LU<S>& f, long off, long m, int w) {
   const long cap = (long)CP_MAXG * CP_THREADS;
   if (m <= 4 * cap) {
-    const int rpt = m <= cap ? 1 : (m <= 2 * cap ? 2 : 4);
+    const int rpt = std::max(ATLG_PANEL_RPT_MIN, m <= cap ? 1 : (m <= 2 * cap ? 2 : 4));
     const int G = (int)((m + (long)CP_THREADS * rpt - 1) / ((long)CP_THREADS * rpt));
     hipLaunchKernelGGL(coop_reset<S>, dim3(1), dim3(1), 0, f.s, f.cb, f.list);
     ATLG_LAUNCHED();
@@ -692,12 +706,12 @@ void laswp(hipStream_t s, long N, S* A, long lda, int K1, int K2, const int* ipi
   }
 }
 
+namespace {
+
+// Scratch and bookkeeping of one factorization stream.
 template <class S>
-void getrf(hipStream_t s, long M, long N, S* A, long lda, int* ipiv, int* dinfo, int gemm_prec) {
-  set_int(s, dinfo, 0);
-  if (M == 0 || N == 0) return;
-  LU<S> f;
-  f.gemm_prec = gemm_prec;
+void init_lu(LU<S>& f, hipStream_t s, long M, long N, S* A, long lda, int* ipiv, int* dinfo,
+             int gemm_prec) {
   f.s = s;
   f.A = A;
   f.lda = lda;
@@ -705,6 +719,9 @@ void getrf(hipStream_t s, long M, long N, S* A, long lda, int* ipiv, int* dinfo,
   f.N = N;
   f.ipiv = ipiv;
   f.dinfo = dinfo;
+  f.gemm_prec = gemm_prec;
+  f.win_lo = 0;
+  f.win_hi = N;
   const long maxblk = (M + STEP_THREADS - 1) / STEP_THREADS + 1;
   const size_t b_cb = (sizeof(CoopBuf<S>) + 255) & ~size_t(255);
   const size_t b_list = 256 + sizeof(int) * (1 + 4 * KC);
@@ -721,7 +738,70 @@ void getrf(hipStream_t s, long M, long N, S* A, long lda, int* ipiv, int* dinfo,
   f.Winv = static_cast<S*>(
       scratch(s, kSlotFactInv, sizeof(S) * std::max<long>(nblk, 1) * kTrsmNB * kTrsmNB));
   f.inv_done.assign(nblk + 1, 0);
-  getrf_rec(f, 0, N);
+}
+
+// Right-looking LU by kLaNB-wide block columns with one step of look-ahead:
+// while the caller's stream applies step k's trailing update to the columns
+// right of block k+1, the auxiliary stream factors block k+1 (updated
+// first).  Each block is factored by ATLAS's recursion (getrf_rec) with its
+// leaves' interchanges confined to the block; the block's interchanges reach
+// the columns to its right after the join and the columns to its left at the
+// end.  Partial pivoting picks from the same updated columns as the plain
+// recursion, so the pivots agree wherever they are well separated.
+template <class S>
+void getrf_lookahead(LU<S>& f0) {
+  hipStream_t s0 = f0.s, s1 = aux_stream();
+  hipEvent_t e_upd = aux_event(0), e_panel = aux_event(1);
+  LU<S> f1;
+  init_lu(f1, s1, f0.M, f0.N, f0.A, f0.lda, f0.ipiv, f0.dinfo, f0.gemm_prec);
+  const long M = f0.M, N = f0.N, mn = std::min(M, N), lda = f0.lda;
+  // block 0 on the caller's stream
+  const long nb0 = std::min(kLaNB, mn);
+  f0.win_lo = 0;
+  f0.win_hi = nb0;
+  getrf_rec(f0, 0, nb0);
+  if (N > nb0) {
+    laswp<S>(s0, N - nb0, at(f0, 0, nb0), lda, 0, (int)nb0, f0.ipiv, 1);
+    node_trsm(f0, 0, nb0, N - nb0);
+  }
+  for (long k = 0; k + kLaNB < mn; k += kLaNB) {
+    const long k1 = k + kLaNB, nb1 = std::min(kLaNB, mn - k1), mrem = M - k1;
+    const long rest = N - k1 - nb1;
+    // block k+1's columns first
+    node_update(f0, mrem, nb1, kLaNB, at(f0, k1, k), at(f0, k, k1), at(f0, k1, k1));
+    ATLG_HIP(hipEventRecord(e_upd, s0));
+    ATLG_HIP(hipStreamWaitEvent(s1, e_upd, 0));
+    f1.win_lo = k1;
+    f1.win_hi = k1 + nb1;
+    getrf_rec(f1, k1, nb1);
+    ATLG_HIP(hipEventRecord(e_panel, s1));
+    // the rest of step k's update, overlapping the panel
+    if (rest > 0)
+      node_update(f0, mrem, rest, kLaNB, at(f0, k1, k), at(f0, k, k1 + nb1),
+                  at(f0, k1, k1 + nb1));
+    ATLG_HIP(hipStreamWaitEvent(s0, e_panel, 0));
+    if (rest > 0) {
+      laswp<S>(s0, rest, at(f0, 0, k1 + nb1), lda, (int)k1, (int)(k1 + nb1), f0.ipiv, 1);
+      node_trsm(f0, k1, nb1, rest);
+    }
+  }
+  // interchanges of blocks 1.. reach the columns to their left
+  for (long k1 = kLaNB; k1 < mn; k1 += kLaNB)
+    laswp<S>(s0, k1, f0.A, lda, (int)k1, (int)std::min(mn, k1 + kLaNB), f0.ipiv, 1);
+}
+
+}  // namespace
+
+template <class S>
+void getrf(hipStream_t s, long M, long N, S* A, long lda, int* ipiv, int* dinfo, int gemm_prec) {
+  set_int(s, dinfo, 0);
+  if (M == 0 || N == 0) return;
+  LU<S> f;
+  init_lu(f, s, M, N, A, lda, ipiv, dinfo, gemm_prec);
+  if (ATLG_LU_LOOKAHEAD && std::min(M, N) >= kLaMin)
+    getrf_lookahead(f);
+  else
+    getrf_rec(f, 0, N);
 }
 
 template <class S>

@@ -24,8 +24,19 @@ namespace {
 
 constexpr int DB = kTrsmNB;    // diagonal block order
 constexpr int LDS_LD = kTriLD;
-constexpr int P_THREADS = kTriThreads;  // 16 x 16 grid of 8 x 8 register tiles
+constexpr int P_THREADS = kTriThreads;
 static_assert(DB == kTriN, "leaf order must match the block inverter");
+
+// Broadcast of lane `src`'s value (uniform src) through the scalar unit.
+__device__ __forceinline__ double lane_value(double v, int src) {
+  const long long u = __double_as_longlong(v);
+  const int lo = __builtin_amdgcn_readlane(static_cast<int>(u), src);
+  const int hi = __builtin_amdgcn_readlane(static_cast<int>(u >> 32), src);
+  return __longlong_as_double((static_cast<long long>(hi) << 32) |
+                              static_cast<unsigned int>(lo));
+}
+
+constexpr int SB = 32;  // sub-block order of the in-workgroup factorization
 
 // Factors the n x n (n <= DB) diagonal block at A and writes inv(L) of the
 // block to Dinv (DB x DB, column-major, zeros above the diagonal).
@@ -33,103 +44,134 @@ static_assert(DB == kTriN, "leaf order must match the block inverter");
 // triangle, handled as L = U^T, so one algorithm serves both.  The other
 // triangle of A is never read or written.
 //
-// Factorization: thread (tr, tc) holds the 8x8 tile at rows 8tr.., columns
-// 8tc.. of the lower triangle in registers; per column, the diagonal owner
-// publishes a_jj, the owners of column j scale it by 1/sqrt(a_jj) (the
-// reciprocal multiply of ATLAS's potrfL closed forms) and publish it, and
-// every lower tile applies the rank-1 update.  Order beyond n is padded with
-// the identity.  The factor is then inverted in LDS (lower_inverse_inplace).
+// Factorization (right-looking, 32-wide sub-blocks, the block held in LDS;
+// order beyond n is padded with the identity):
+//   1. wave 0 factors the 32 x 32 diagonal sub-block in registers, lane i
+//      holding row i; the column of each step reaches the other lanes by
+//      v_readlane, so no barrier is needed inside the sub-block (the
+//      reciprocal multiply of ATLAS's potrfL closed forms);
+//   2. the rows below solve X L11^T = A21 (one row per thread, L11 read
+//      from LDS);
+//   3. the trailing lower triangle is updated A22 -= X X^T (4 x 4 register
+//      tiles).
+// The factor is then inverted in LDS (lower_inverse_inplace).
 __global__ void __launch_bounds__(P_THREADS)
 potf2_inv_kernel(int n, double* __restrict__ A, long lda, int lower, int* __restrict__ dinfo,
                  int info_off, double* __restrict__ Dinv) {
   if (*dinfo != 0) return;
   __shared__ double S[DB * LDS_LD];
-  __shared__ double col[DB];
-  __shared__ double dsh;
-  const int t = threadIdx.x, tr = t >> 4, tc = t & 15;
-  const bool act = tr >= tc;
-  double a[8][8];
-#pragma unroll
-  for (int r = 0; r < 8; r++)
-#pragma unroll
-    for (int c = 0; c < 8; c++) {
-      const int i = 8 * tr + r, k = 8 * tc + c;
-      double v = 0.0;
-      if (act && i >= k) {
-        if (i < n && k < n) v = lower ? A[i + (size_t)k * lda] : A[k + (size_t)i * lda];
-        else if (i == k) v = 1.0;
-      }
-      a[r][c] = v;
+  __shared__ int fail_sh;
+  const int t = threadIdx.x;
+  for (int idx = t; idx < DB * DB; idx += P_THREADS) {
+    const int i = idx % DB, k = idx / DB;
+    double v = 0.0;
+    if (i >= k) {
+      if (i < n && k < n) v = lower ? A[i + (size_t)k * lda] : A[k + (size_t)i * lda];
+      else if (i == k) v = 1.0;
     }
+    S[i * LDS_LD + k] = v;
+  }
+  if (t == 0) fail_sh = -1;
+  __syncthreads();
 
   int fail = -1;
-  for (int j0 = 0; j0 < DB / 8 && fail < 0; j0++) {
+  for (int kb = 0; kb < DB; kb += SB) {
+    // 1. diagonal sub-block, wave 0
+    if (t < 64) {
+      const int i = t;  // lanes 32..63 carry zeros
+      double x[SB];
 #pragma unroll
-    for (int jj = 0; jj < 8; jj++) {
-      const int j = 8 * j0 + jj;
-      if (tr == j0 && tc == j0) dsh = a[jj][jj];
-      __syncthreads();
-      const double d = dsh;
-      if (!(d > 0.0)) {  // same value in every thread: uniform exit
-        fail = j;
-        break;
-      }
-      const double l = sqrt(d), rinv = 1.0 / l;
-      if (tc == j0 && tr >= j0) {
+      for (int c = 0; c < SB; c++)
+        x[c] = (i < SB && c <= i) ? S[(kb + i) * LDS_LD + kb + c] : 0.0;
+      int f = -1;  // uniform: every lane reads the same pivots
 #pragma unroll
-        for (int r = 0; r < 8; r++) {
-          const int i = 8 * tr + r;
-          if (i == j) {
-            a[r][jj] = l;
-          } else if (i > j) {
-            a[r][jj] *= rinv;
-            col[i] = a[r][jj];
+      for (int j = 0; j < SB; j++) {
+        const double d = lane_value(x[j], j);
+        if (f < 0 && !(d > 0.0)) f = j;
+        if (f < 0) {
+          const double l = sqrt(d), rinv = 1.0 / l;
+          x[j] = (i == j) ? l : ((i > j) ? x[j] * rinv : x[j]);
+#pragma unroll
+          for (int c = j + 1; c < SB; c++) {
+            const double lc = lane_value(x[j], c);  // L[c][j]
+            x[c] = (i >= c) ? x[c] - x[j] * lc : x[c];
           }
         }
       }
-      __syncthreads();
-      if (act && tc >= j0) {
-        double ci[8], ck[8];
+      if (i < SB) {
 #pragma unroll
-        for (int r = 0; r < 8; r++) ci[r] = col[8 * tr + r];
-#pragma unroll
-        for (int c = 0; c < 8; c++) ck[c] = col[8 * tc + c];
-#pragma unroll
-        for (int r = 0; r < 8; r++)
-#pragma unroll
-          for (int c = 0; c < 8; c++) {
-            const int i = 8 * tr + r, k = 8 * tc + c;
-            if (k > j && i >= k) a[r][c] -= ci[r] * ck[c];
-          }
+        for (int c = 0; c < SB; c++)
+          if (c <= i) S[(kb + i) * LDS_LD + kb + c] = x[c];
       }
+      if (t == 0 && f >= 0) fail_sh = kb + f;
     }
+    __syncthreads();
+    fail = fail_sh;
+    if (fail >= 0) break;
+    const int r0 = kb + SB, nrow = DB - r0;
+    if (nrow <= 0) break;
+    // 2. rows below: X[i][:] L11^T = A21[i][:]
+    if (t < nrow) {
+      const int i = r0 + t;
+      double x[SB];
+#pragma unroll
+      for (int c = 0; c < SB; c++) x[c] = S[i * LDS_LD + kb + c];
+#pragma unroll
+      for (int c = 0; c < SB; c++) {
+        double acc = x[c];
+#pragma unroll
+        for (int k = 0; k < c; k++) acc -= x[k] * S[(kb + c) * LDS_LD + kb + k];
+        x[c] = acc / S[(kb + c) * LDS_LD + kb + c];
+      }
+#pragma unroll
+      for (int c = 0; c < SB; c++) S[i * LDS_LD + kb + c] = x[c];
+    }
+    __syncthreads();
+    // 3. trailing update of the lower triangle, 4 x 4 tiles
+    const int tiles = nrow / 4, ntile = tiles * (tiles + 1) / 2;
+    for (int u = t; u < ntile; u += P_THREADS) {
+      int ti = (int)((sqrtf(8.0f * u + 1.0f) - 1.0f) * 0.5f);
+      while (ti * (ti + 1) / 2 > u) ti--;
+      while ((ti + 1) * (ti + 2) / 2 <= u) ti++;
+      const int tj = u - ti * (ti + 1) / 2;
+      double acc[4][4] = {};
+#pragma unroll 8
+      for (int c = 0; c < SB; c++) {
+        double xi[4], xj[4];
+#pragma unroll
+        for (int a = 0; a < 4; a++) {
+          xi[a] = S[(r0 + 4 * ti + a) * LDS_LD + kb + c];
+          xj[a] = S[(r0 + 4 * tj + a) * LDS_LD + kb + c];
+        }
+#pragma unroll
+        for (int a = 0; a < 4; a++)
+#pragma unroll
+          for (int b = 0; b < 4; b++) acc[a][b] += xi[a] * xj[b];
+      }
+#pragma unroll
+      for (int a = 0; a < 4; a++)
+#pragma unroll
+        for (int b = 0; b < 4; b++) {
+          const int i = r0 + 4 * ti + a, k = r0 + 4 * tj + b;
+          if (i >= k) S[i * LDS_LD + k] -= acc[a][b];
+        }
+    }
+    __syncthreads();
   }
-  if (fail >= 0) {
-    if (t == 0) *dinfo = info_off + fail + 1;
-  }
+  if (fail >= 0 && t == 0) *dinfo = info_off + fail + 1;
   // write back (on failure: the partially factored block, as ATLAS leaves it)
-#pragma unroll
-  for (int r = 0; r < 8; r++)
-#pragma unroll
-    for (int c = 0; c < 8; c++) {
-      const int i = 8 * tr + r, k = 8 * tc + c;
-      if (act && i >= k && i < n && k < n) {
-        if (lower) A[i + (size_t)k * lda] = a[r][c];
-        else A[k + (size_t)i * lda] = a[r][c];
-      }
+  for (int idx = t; idx < DB * DB; idx += P_THREADS) {
+    const int i = idx % DB, k = idx / DB;
+    if (i >= k && i < n && k < n) {
+      if (lower) A[i + (size_t)k * lda] = S[i * LDS_LD + k];
+      else A[k + (size_t)i * lda] = S[i * LDS_LD + k];
     }
+  }
   if (fail >= 0 || Dinv == nullptr) return;
 
-  // ---- inverse of the factor ----
-#pragma unroll
-  for (int r = 0; r < 8; r++)
-#pragma unroll
-    for (int c = 0; c < 8; c++) {
-      const int i = 8 * tr + r, k = 8 * tc + c;
-      S[i * LDS_LD + k] = (act && i >= k) ? a[r][c] : 0.0;
-    }
+  // ---- inverse of the factor (upper part of S is zero) ----
   __syncthreads();
-  lower_inverse_inplace(S);
+  lower_inverse_inplace(S, n);
   for (int idx = t; idx < DB * DB; idx += P_THREADS) {
     const int i = idx % DB, k = idx / DB;
     Dinv[idx] = (i >= k) ? S[i * LDS_LD + k] : 0.0;

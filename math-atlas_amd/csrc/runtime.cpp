@@ -47,6 +47,28 @@ hipStream_t library_stream() {
   return st;
 }
 
+namespace {
+struct Aux {
+  hipStream_t stream = nullptr;
+  hipEvent_t ev[2] = {nullptr, nullptr};
+};
+std::map<int, Aux> g_aux;
+Aux& aux_for_device() {
+  int dev = 0;
+  ATLG_HIP(hipGetDevice(&dev));
+  std::lock_guard<std::mutex> lk(g_mu);
+  Aux& a = g_aux[dev];
+  if (!a.stream) {
+    ATLG_HIP(hipStreamCreateWithFlags(&a.stream, hipStreamNonBlocking));
+    for (auto& e : a.ev) ATLG_HIP(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+  }
+  return a;
+}
+}  // namespace
+
+hipStream_t aux_stream() { return aux_for_device().stream; }
+hipEvent_t aux_event(int i) { return aux_for_device().ev[i & 1]; }
+
 void* scratch(hipStream_t s, Slot slot, size_t bytes) {
   if (bytes == 0) bytes = 256;
   std::lock_guard<std::mutex> lk(g_mu);

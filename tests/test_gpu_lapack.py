@@ -62,6 +62,21 @@ def test_getrf_panel_paths(G):
         lu_case(G, m, n)
 
 
+def test_getrf_large_ragged(G):
+    # orders >= 2048 in block-column steps of 256 with ragged ends, both
+    # rectangles, and a zero column past the first block (info = 1001)
+    for (m, n) in ((2600, 2300), (2300, 2900)):
+        lu_case(G, m, n, gap=3)
+    n = 2300
+    a0 = O.gegen(n, n, 777)
+    a0[:, 1000] = 0.0
+    lu = a0.copy(order="F")
+    ipiv = np.zeros(n, dtype=np.int32)
+    assert G.clapack_dgetrf(G.ColMajor, n, n, lu, n, ipiv) == 1001
+    assert np.all(np.isfinite(lu))
+    assert O.lu_resid(n, n, a0, lu, ipiv) <= 100
+
+
 def test_getrf_known_pivots(G):
     # A = P^T L U with |L_ij| < 0.45/n and diag(U) in [1, 2]: at every step the
     # unit-diagonal row beats every competitor by a factor > n (and L, U are
