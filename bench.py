@@ -238,7 +238,7 @@ def bench_ir(torch, G, n):
             out["ir_dgesv_fp64_ms"] = ms64
             out["ir_dgesv_fp64_berr"] = berr()
             out["ir_dgetrf_fp64_tflops"] = (2.0 * n ** 3 / 3) / ms64 / 1e9
-        for lp, name in ((0, "fp32"), (1, "bf16")):
+        for lp, name in ((0, "fp32"), (1, "bf16"), (2, "bf16x6")):
             ms = timed(torch, lambda: runmix(lp), 1) - ms_copy
             out["ir_k%s_%s_ms" % (tag, name)] = ms
             out["ir_k%s_%s_iter" % (tag, name)] = its[lp]

@@ -96,12 +96,15 @@ int atlg_dlogdet_chol_adjoint(long N, const double *L, long ldl,
  * ATL_sgetrs; AtlasBase/Clint/atlas-lp.base:1497-1720) ------------------
  * atlg_sgemm: as atlg_dgemm in FP32 (FP32 MFMA).
  * atlg_sgemm_bf16: C = alpha*bf16(A)*bf16(B) + beta*C, NoTrans/NoTrans,
- *   operands rounded to bfloat16 (nearest even), FP32 accumulation and C.
+ *   operands rounded to bfloat16 (nearest even), FP32 accumulation and C;
+ *   split != 0: each operand as hi + mid + lo BF16 terms (all 24 bits),
+ *   the six products above FP32 resolution (six BF16 MFMAs, FP32
+ *   accuracy).
  * atlg_sgetrf: as atlg_dgetrf in FP32; gemm_prec 0 = FP32 trailing updates,
- *   1 = BF16 trailing updates with FP32 accumulation.
+ *   1 = BF16 trailing updates with FP32 accumulation, 2 = split BF16 (x6).
  * atlg_sgetrs: as atlg_dgetrs in FP32.
  * atlg_dsgesv: LAPACK dsgesv semantics.  Factors float(A) with atlg_sgetrf
- *   (gemm_prec = lowprec), refines X with FP64 residuals B - A X until
+ *   (gemm_prec = lowprec: 0 FP32, 1 BF16, 2 split BF16), refines X with FP64 residuals B - A X until
  *   ||r_j||_inf < ||x_j||_inf ||A||_inf eps sqrt(N) for every column j (at
  *   most itermax corrections; itermax <= 0 means 30).  *iter (host) = the
  *   corrections used, or -2 (A overflows FP32), -3 (FP32 factor singular),
@@ -113,7 +116,7 @@ int atlg_sgemm(int TA, int TB, long M, long N, long K, float alpha,
                float *C, long ldc, atlg_stream_t stream);
 int atlg_sgemm_bf16(long M, long N, long K, float alpha, const float *A,
                     long lda, const float *B, long ldb, float beta, float *C,
-                    long ldc, atlg_stream_t stream);
+                    long ldc, int split, atlg_stream_t stream);
 int atlg_sgetrf(long M, long N, float *A, long lda, int *ipiv, int *dinfo,
                 int gemm_prec, atlg_stream_t stream);
 int atlg_sgetrs(int Trans, long N, long NRHS, const float *A, long lda,

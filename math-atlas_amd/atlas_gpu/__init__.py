@@ -77,7 +77,7 @@ _SIGS = {
     "atlg_sgemm": (_i, [_i, _i, _l, _l, _l, ctypes.c_float, _v, _l, _v, _l, ctypes.c_float, _v,
                         _l, _v]),
     "atlg_sgemm_bf16": (_i, [_l, _l, _l, ctypes.c_float, _v, _l, _v, _l, ctypes.c_float, _v, _l,
-                             _v]),
+                             _i, _v]),
     "atlg_sgetrf": (_i, [_l, _l, _v, _l, _v, _v, _i, _v]),
     "atlg_sgetrs": (_i, [_i, _l, _l, _v, _l, _v, _v, _l, _v]),
     "atlg_dsgesv": (_i, [_l, _l, _v, _l, _v, _v, _l, _v, _l, _v, _i, _i, _ip, _v]),
@@ -395,9 +395,9 @@ def sgemm(ta, tb, m, n, k, alpha, a, lda, b, ldb, beta, c, ldc, stream=None):
                             ldc, _stream(stream)), "atlg_sgemm")
 
 
-def sgemm_bf16(m, n, k, alpha, a, lda, b, ldb, beta, c, ldc, stream=None):
+def sgemm_bf16(m, n, k, alpha, a, lda, b, ldb, beta, c, ldc, split=0, stream=None):
     _check(lib().atlg_sgemm_bf16(m, n, k, alpha, _ptr(a), lda, _ptr(b), ldb, beta, _ptr(c), ldc,
-                                 _stream(stream)), "atlg_sgemm_bf16")
+                                 split, _stream(stream)), "atlg_sgemm_bf16")
 
 
 def sgetrf(m, n, a, lda, ipiv, info, gemm_prec=0, stream=None):

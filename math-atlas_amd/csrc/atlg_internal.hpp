@@ -76,7 +76,8 @@ enum Slot : int {
   kSlotBatchWork = 12,  // FP32 factors and right-hand sides of the mixed solver
   kSlotMixed = 13,      // FP32 factor, residual partials and norms of dsgesv
   kSlotBf16 = 14,       // BF16 operand copies of the low-precision update
-  kNumSlots = 15
+  kSlotTrsv = 15,       // ready flags of the few-RHS triangular solve
+  kNumSlots = 16
 };
 void* scratch(hipStream_t s, Slot slot, size_t bytes);
 
@@ -119,8 +120,9 @@ template <class S>
 void laswp(hipStream_t s, long N, S* A, long lda, int K1, int K2, const int* ipiv, int inci);
 
 // Precision of getrf's trailing-update GEMMs: the scalar type's own MFMA,
-// or (float only) BF16 operands with FP32 accumulation.
-enum GemmPrec : int { kGemmNative = 0, kGemmBf16 = 1 };
+// or (float only) BF16 operands with FP32 accumulation, single or split
+// (hi + mid + lo, six products: FP32 accuracy) operands.
+enum GemmPrec : int { kGemmNative = 0, kGemmBf16 = 1, kGemmBf16x6 = 2 };
 template <class S>
 void getrf(hipStream_t s, long M, long N, S* A, long lda, int* ipiv, int* dinfo,
            int gemm_prec = kGemmNative);
@@ -168,9 +170,11 @@ void dsposv_batched(hipStream_t s, int nsys, const int* n, double* const* A, con
                     int nrhs, int* info, int* iter, int itermax, const BatchTune& tune);
 
 // C(f32) = alpha * bf16(A) * bf16(B) + beta * C, A M x K and B K x N
-// column-major FP32 (NoTrans, NoTrans), FP32 accumulation.
+// column-major FP32 (NoTrans, NoTrans), FP32 accumulation.  split: each
+// operand as hi + mid + lo BF16 terms, the six products above FP32
+// resolution (FP32 accuracy).
 void gemm_bf16(hipStream_t s, long M, long N, long K, float alpha, const float* A, long lda,
-               const float* B, long ldb, float beta, float* C, long ldc);
+               const float* B, long ldb, float beta, float* C, long ldc, bool split = false);
 
 // Mixed-precision dgesv (LAPACK dsgesv semantics): FP32 LU (lowprec != 0:
 // BF16 trailing updates), FP64 residual refinement, FP64 fallback.  *iter_out

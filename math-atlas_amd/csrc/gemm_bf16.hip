@@ -5,12 +5,22 @@
 // (AtlasBase/Clint/atlas-lp.base:1497-1618) when the factor is computed with
 // BF16 products.
 //
+// Split variant (NPROD = 6): each FP32 operand x is carried as three BF16
+// terms, hi = bf16(x), mid = bf16(x - hi), lo = bf16(x - hi - mid), which
+// hold all 24 bits of its significand, and the product is accumulated as
+// hi*hi + hi*mid + mid*hi + mid*mid + hi*lo + lo*hi (the terms below FP32
+// resolution are dropped: "bf16x6" of Henry, Tang and Heinecke, "Leveraging
+// the bfloat16 artificial intelligence datatype for higher-precision
+// computations", ARITH 2019).  Six BF16 MFMAs per tile step give FP32
+// accuracy.
+//
 // The operands are first converted into K-contiguous BF16 copies (A
 // transposed through LDS; B's columns are already K-contiguous), with K
 // padded to the tile depth with zeros.  The GEMM then streams 16-byte
 // vectors of 8 BF16 into double-buffered LDS tiles and runs
 // v_mfma_f32_16x16x32_bf16: 128x128 C per 256-thread workgroup, 64x64 per
-// wave, K consumed 64 at a time.
+// wave, K consumed 64 at a time (32 for the split variant, whose three
+// tiles per operand share the LDS).
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -24,10 +34,9 @@ typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 typedef float f4 __attribute__((ext_vector_type(4)));
 typedef unsigned short us8 __attribute__((ext_vector_type(8)));
 
-constexpr int BM = 128, BN = 128, BK = 64, NT = 256;
+constexpr int BM = 128, BN = 128, NT = 256;
 constexpr int WM = 64, WN = 64, TM = WM / 16, TN = WN / 16;
-constexpr int LDK = BK + 8;  // LDS row pitch in BF16 (16-byte pad)
-constexpr int KPAD = BK;     // K of the converted copies is a multiple of this
+constexpr int KPAD = 64;     // K of the converted copies is a multiple of this
 
 // Register layout of v_mfma_f32_16x16x32_bf16: lane l supplies
 // X1[l&15][8(l>>4) .. 8(l>>4)+7] and X2[8(l>>4) .. +7][l&15]; D[i][j] is in
@@ -37,10 +46,22 @@ constexpr int KPAD = BK;     // K of the converted copies is a multiple of this
 // dst[x][k] (k contiguous, pitch ldd, x < X, k < Kp) = bf16(src(x, k)) where
 // src(x, k) = S[x + k*lds] (TRANS: x contiguous in the source) or
 // S[k + x*lds]; zero for k >= K.
+// With Dmid/Dlo != nullptr also writes the remainders: mid = bf16(x - hi),
+// lo = bf16(x - hi - mid) (each difference is exact in FP32).
+__device__ inline void split3(float v, __bf16* hi, __bf16* mid, __bf16* lo) {
+  const __bf16 h = (__bf16)v;
+  const float r1 = v - (float)h;
+  const __bf16 m = (__bf16)r1;
+  *hi = h;
+  *mid = m;
+  *lo = (__bf16)(r1 - (float)m);
+}
+
 template <bool TRANS>
 __global__ void __launch_bounds__(256)
 to_bf16_kernel(long X, long K, long Kp, const float* __restrict__ S, long lds,
-               __bf16* __restrict__ D, long ldd) {
+               __bf16* __restrict__ D, long ldd, __bf16* __restrict__ Dmid,
+               __bf16* __restrict__ Dlo) {
   if (TRANS) {
     __shared__ float tile[64][65];
     const long x0 = (long)blockIdx.x * 64, k0 = (long)blockIdx.y * 64;
@@ -52,49 +73,83 @@ to_bf16_kernel(long X, long K, long Kp, const float* __restrict__ S, long lds,
     __syncthreads();
     for (int r = ty; r < 64; r += 4) {
       const long x = x0 + r, k = k0 + tx;
-      if (x < X && k < Kp) D[x * ldd + k] = (__bf16)tile[tx][r];
+      if (x < X && k < Kp) {
+        const float v = tile[tx][r];
+        if (Dmid) {
+          __bf16 h, m, l;
+          split3(v, &h, &m, &l);
+          D[x * ldd + k] = h;
+          Dmid[x * ldd + k] = m;
+          Dlo[x * ldd + k] = l;
+        } else {
+          D[x * ldd + k] = (__bf16)v;
+        }
+      }
     }
   } else {
     const long x = blockIdx.x;
-    for (long k = threadIdx.x; k < Kp; k += blockDim.x)
-      D[x * ldd + k] = (k < K) ? (__bf16)S[k + x * lds] : (__bf16)0.0f;
+    for (long k = threadIdx.x; k < Kp; k += blockDim.x) {
+      const float v = (k < K) ? S[k + x * lds] : 0.0f;
+      if (Dmid) {
+        __bf16 h, m, l;
+        split3(v, &h, &m, &l);
+        D[x * ldd + k] = h;
+        Dmid[x * ldd + k] = m;
+        Dlo[x * ldd + k] = l;
+      } else {
+        D[x * ldd + k] = (__bf16)v;
+      }
+    }
   }
 }
 
-// C[m][n] = alpha * sum_k At[m][k] * Bt[n][k] + beta * C[m][n]
-__global__ void __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(2, 2)))
-gemm_bf16_kernel(int M, int N, int Kp, float alpha, const __bf16* __restrict__ At, long lda,
-                 const __bf16* __restrict__ Bt, long ldb, float beta, float* __restrict__ C,
+// C[m][n] = alpha * sum_k At[m][k] * Bt[n][k] + beta * C[m][n]; with
+// NPROD = 6 each operand comes as hi/mid/lo copies (A0/A1/A2, B0/B1/B2).
+template <int NPROD>
+__global__ void __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(1, 2)))
+gemm_bf16_kernel(int M, int N, int Kp, float alpha, const __bf16* __restrict__ A0,
+                 const __bf16* __restrict__ A1, const __bf16* __restrict__ A2, long lda,
+                 const __bf16* __restrict__ B0, const __bf16* __restrict__ B1,
+                 const __bf16* __restrict__ B2, long ldb, float beta, float* __restrict__ C,
                  long ldc) {
-  __shared__ __attribute__((aligned(16))) __bf16 As[2][BM * LDK];
-  __shared__ __attribute__((aligned(16))) __bf16 Bs[2][BN * LDK];
+  constexpr int NOP = (NPROD == 1) ? 1 : 3;  // operand copies per side
+  constexpr int BK = (NPROD == 1) ? 64 : 32;
+  constexpr int LDK = BK + 8;                // LDS row pitch in BF16 (16-byte pad)
+  constexpr int VPR = BK / 8;                // 16-byte vectors per row
+  constexpr int NV = BM * VPR / NT;          // vectors per thread per operand copy
+  __shared__ __attribute__((aligned(16))) __bf16 As[2][NOP][BM * LDK];
+  __shared__ __attribute__((aligned(16))) __bf16 Bs[2][NOP][BN * LDK];
   const int tilesM = (M + BM - 1) / BM;
   const int tm = blockIdx.x % tilesM, tn = blockIdx.x / tilesM;
   const int m0 = tm * BM, n0 = tn * BN;
   const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
   const int wm0 = (wave >> 1) * WM, wn0 = (wave & 1) * WN;
+  const __bf16* Ap[3] = {A0, A1, A2};
+  const __bf16* Bp[3] = {B0, B1, B2};
 
-  // each thread moves 4 vectors of 8 BF16 per operand tile: row = e >> 3,
-  // k-vector = e & 7 for e = t + 256 q
-  us8 ra[4], rb[4];
+  us8 ra[NOP][NV], rb[NOP][NV];
   auto gload = [&](int k0) {
 #pragma unroll
-    for (int q = 0; q < 4; q++) {
-      const int e = t + NT * q, row = e >> 3, kv = (e & 7) * 8;
-      const int gm = m0 + row, gn = n0 + row;
-      ra[q] = (gm < M) ? *reinterpret_cast<const us8*>(At + (size_t)gm * lda + k0 + kv)
-                       : us8{0, 0, 0, 0, 0, 0, 0, 0};
-      rb[q] = (gn < N) ? *reinterpret_cast<const us8*>(Bt + (size_t)gn * ldb + k0 + kv)
-                       : us8{0, 0, 0, 0, 0, 0, 0, 0};
-    }
+    for (int o = 0; o < NOP; o++)
+#pragma unroll
+      for (int q = 0; q < NV; q++) {
+        const int e = t + NT * q, row = e / VPR, kv = (e % VPR) * 8;
+        const int gm = m0 + row, gn = n0 + row;
+        ra[o][q] = (gm < M) ? *reinterpret_cast<const us8*>(Ap[o] + (size_t)gm * lda + k0 + kv)
+                            : us8{0, 0, 0, 0, 0, 0, 0, 0};
+        rb[o][q] = (gn < N) ? *reinterpret_cast<const us8*>(Bp[o] + (size_t)gn * ldb + k0 + kv)
+                            : us8{0, 0, 0, 0, 0, 0, 0, 0};
+      }
   };
   auto lstore = [&](int buf) {
 #pragma unroll
-    for (int q = 0; q < 4; q++) {
-      const int e = t + NT * q, row = e >> 3, kv = (e & 7) * 8;
-      *reinterpret_cast<us8*>(&As[buf][row * LDK + kv]) = ra[q];
-      *reinterpret_cast<us8*>(&Bs[buf][row * LDK + kv]) = rb[q];
-    }
+    for (int o = 0; o < NOP; o++)
+#pragma unroll
+      for (int q = 0; q < NV; q++) {
+        const int e = t + NT * q, row = e / VPR, kv = (e % VPR) * 8;
+        *reinterpret_cast<us8*>(&As[buf][o][row * LDK + kv]) = ra[o][q];
+        *reinterpret_cast<us8*>(&Bs[buf][o][row * LDK + kv]) = rb[o][q];
+      }
   };
 
   f4 acc[TM][TN];
@@ -114,18 +169,31 @@ gemm_bf16_kernel(int M, int N, int Kp, float alpha, const __bf16* __restrict__ A
     if (more) gload((kt + 1) * BK);
 #pragma unroll
     for (int ks = 0; ks < BK; ks += 32) {
-      bf16x8 af[TM], bfr[TN];
+      bf16x8 af[NOP][TM], bfr[NOP][TN];
 #pragma unroll
-      for (int i = 0; i < TM; i++)
-        af[i] = *reinterpret_cast<const bf16x8*>(&As[cur][(wm0 + i * 16 + fx) * LDK + ks + fk]);
+      for (int o = 0; o < NOP; o++) {
 #pragma unroll
-      for (int j = 0; j < TN; j++)
-        bfr[j] = *reinterpret_cast<const bf16x8*>(&Bs[cur][(wn0 + j * 16 + fx) * LDK + ks + fk]);
-#pragma unroll
-      for (int i = 0; i < TM; i++)
+        for (int i = 0; i < TM; i++)
+          af[o][i] =
+              *reinterpret_cast<const bf16x8*>(&As[cur][o][(wm0 + i * 16 + fx) * LDK + ks + fk]);
 #pragma unroll
         for (int j = 0; j < TN; j++)
-          acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(bfr[j], af[i], acc[i][j], 0, 0, 0);
+          bfr[o][j] =
+              *reinterpret_cast<const bf16x8*>(&Bs[cur][o][(wn0 + j * 16 + fx) * LDK + ks + fk]);
+      }
+#pragma unroll
+      for (int i = 0; i < TM; i++)
+#pragma unroll
+        for (int j = 0; j < TN; j++) {
+          if (NPROD == 6) {  // small terms first: lo*hi, hi*lo, mid*mid, mid*hi, hi*mid
+            acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(bfr[2][j], af[0][i], acc[i][j], 0, 0, 0);
+            acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(bfr[0][j], af[2][i], acc[i][j], 0, 0, 0);
+            acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(bfr[1][j], af[1][i], acc[i][j], 0, 0, 0);
+            acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(bfr[1][j], af[0][i], acc[i][j], 0, 0, 0);
+            acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(bfr[0][j], af[1][i], acc[i][j], 0, 0, 0);
+          }
+          acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(bfr[0][j], af[0][i], acc[i][j], 0, 0, 0);
+        }
     }
     if (more) lstore(cur ^ 1);
     __syncthreads();
@@ -149,7 +217,7 @@ gemm_bf16_kernel(int M, int N, int Kp, float alpha, const __bf16* __restrict__ A
 }  // namespace
 
 void gemm_bf16(hipStream_t s, long M, long N, long K, float alpha, const float* A, long lda,
-               const float* B, long ldb, float beta, float* C, long ldc) {
+               const float* B, long ldb, float beta, float* C, long ldc, bool split) {
   if (M <= 0 || N <= 0) return;
   if (K <= 0 || alpha == 0.0f) {
     scale<float>(s, M, N, beta, C, ldc);
@@ -157,19 +225,31 @@ void gemm_bf16(hipStream_t s, long M, long N, long K, float alpha, const float* 
   }
   const long Kp = (K + KPAD - 1) / KPAD * KPAD;
   const size_t ba = ((size_t)M * Kp * sizeof(__bf16) + 255) & ~size_t(255);
-  const size_t bb = (size_t)N * Kp * sizeof(__bf16);
-  char* w = static_cast<char*>(scratch(s, kSlotBf16, ba + bb));
-  __bf16* At = reinterpret_cast<__bf16*>(w);
-  __bf16* Bt = reinterpret_cast<__bf16*>(w + ba);
+  const size_t bb = ((size_t)N * Kp * sizeof(__bf16) + 255) & ~size_t(255);
+  const int copies = split ? 3 : 1;
+  char* w = static_cast<char*>(scratch(s, kSlotBf16, copies * (ba + bb)));
+  __bf16* Ac[3];
+  __bf16* Bc[3];
+  for (int c = 0; c < 3; c++) {
+    Ac[c] = reinterpret_cast<__bf16*>(w + (size_t)(c < copies ? c : 0) * (ba + bb));
+    Bc[c] = reinterpret_cast<__bf16*>(w + (size_t)(c < copies ? c : 0) * (ba + bb) + ba);
+  }
   hipLaunchKernelGGL(to_bf16_kernel<true>, dim3((unsigned)((M + 63) / 64), (unsigned)(Kp / 64)),
-                     dim3(256), 0, s, M, K, Kp, A, lda, At, Kp);
+                     dim3(256), 0, s, M, K, Kp, A, lda, Ac[0], Kp, split ? Ac[1] : nullptr,
+                     split ? Ac[2] : nullptr);
   ATLG_LAUNCHED();
   hipLaunchKernelGGL(to_bf16_kernel<false>, dim3((unsigned)N), dim3(256), 0, s, N, K, Kp, B, ldb,
-                     Bt, Kp);
+                     Bc[0], Kp, split ? Bc[1] : nullptr, split ? Bc[2] : nullptr);
   ATLG_LAUNCHED();
   const long tiles = ((M + BM - 1) / BM) * ((N + BN - 1) / BN);
-  hipLaunchKernelGGL(gemm_bf16_kernel, dim3((unsigned)tiles), dim3(NT), 0, s, (int)M, (int)N,
-                     (int)Kp, alpha, At, Kp, Bt, Kp, beta, C, ldc);
+  if (split)
+    hipLaunchKernelGGL(gemm_bf16_kernel<6>, dim3((unsigned)tiles), dim3(NT), 0, s, (int)M, (int)N,
+                       (int)Kp, alpha, Ac[0], Ac[1], Ac[2], Kp, Bc[0], Bc[1], Bc[2], Kp, beta, C,
+                       ldc);
+  else
+    hipLaunchKernelGGL(gemm_bf16_kernel<1>, dim3((unsigned)tiles), dim3(NT), 0, s, (int)M, (int)N,
+                       (int)Kp, alpha, Ac[0], Ac[0], Ac[0], Kp, Bc[0], Bc[0], Bc[0], Kp, beta, C,
+                       ldc);
   ATLG_LAUNCHED();
 }
 

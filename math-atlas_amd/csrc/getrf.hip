@@ -51,7 +51,7 @@ constexpr int CP_MAXG = 64;       // workgroups of the cooperative panel
 constexpr int STEP_THREADS = 256;
 constexpr int KC = 256;           // interchanges composed per laswp chunk
 constexpr int CW = 16;            // columns per laswp workgroup
-constexpr long kSpinLimit = 1L << 20;  // barrier polls (~0.1 s) before giving up
+constexpr long kSpinLimit = 1L << 20;  // barrier polls (each one round trip) before giving up
 // Fewest panel rows per thread of the cooperative panel (1, 2 or 4): more
 // rows per thread means fewer workgroups exchanging candidates per column.
 #ifndef ATLG_PANEL_RPT_MIN
@@ -294,16 +294,19 @@ __device__ bool grid_sync(CoopBuf<S>* cb, unsigned target, int* verdict) {
   __syncthreads();
   if (threadIdx.x == 0) {
     __hip_atomic_fetch_add(&cb->bar, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    // Each poll is one round trip to the coherence point, which paces the
+    // loop; the error word (another workgroup gave up) is read only every
+    // 256 polls so that the common case costs a single round trip.
+    bool ok = true;
     long spins = 0;
     while (__hip_atomic_load(&cb->bar, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < target) {
-      if (ld_agent(&cb->err)) break;
-      if (++spins > kSpinLimit) {
+      if ((++spins & 255) == 0 && (ld_agent(&cb->err) || spins > kSpinLimit)) {
         st_agent(&cb->err, 1);
+        ok = false;
         break;
       }
-      __builtin_amdgcn_s_sleep(1);
     }
-    *verdict = (ld_agent(&cb->err) == 0);
+    *verdict = ok;
   }
   __syncthreads();
   return *verdict != 0;
@@ -350,6 +353,23 @@ panel_coop(int M, int w, int off, S* __restrict__ P, long lda, int* __restrict__
       }
     }
     block_argmax(key, idx, sk, si);
+    if (G == 1) {
+      // the whole panel is in this workgroup: exchange through LDS only
+#pragma unroll
+      for (int q = 0; q < RPT; q++) {
+        const int i = g + q * GT;
+        if (i == idx) {
+#pragma unroll
+          for (int c = 0; c < PW; c++) crows[c] = (c < w) ? v[q][c] : S(0);
+        }
+        if (i == j) {
+#pragma unroll
+          for (int c = 0; c < PW; c++) jrow[c] = (c < w) ? v[q][c] : S(0);
+        }
+      }
+      if (t == 0) winner = idx;
+      __syncthreads();
+    } else {
     bool wrote = false;
 #pragma unroll
     for (int q = 0; q < RPT; q++) {
@@ -395,6 +415,7 @@ panel_coop(int M, int w, int off, S* __restrict__ P, long lda, int* __restrict__
       if (t == 0) winner = ii;
     }
     __syncthreads();
+    }  // G > 1
     // rows j..M-1 always hold a candidate, so p is in [j, M); the clamp only
     // keeps a corrupted exchange from indexing outside the panel
     const int p = (winner >= j && winner < M) ? winner : j;
@@ -566,8 +587,9 @@ inline void node_update(LU<double>& f, long m, long n, long k, const double* A, 
 }
 inline void node_update(LU<float>& f, long m, long n, long k, const float* A, float* B,
                         float* C) {
-  if (f.gemm_prec == kGemmBf16)
-    gemm_bf16(f.s, m, n, k, -1.0f, A, f.lda, B, f.lda, 1.0f, C, f.lda);
+  if (f.gemm_prec == kGemmBf16 || f.gemm_prec == kGemmBf16x6)
+    gemm_bf16(f.s, m, n, k, -1.0f, A, f.lda, B, f.lda, 1.0f, C, f.lda,
+              f.gemm_prec == kGemmBf16x6);
   else
     gemm<float>(f.s, kNoTrans, kNoTrans, m, n, k, -1.0f, A, f.lda, B, f.lda, 1.0f, C, f.lda);
 }

@@ -203,7 +203,8 @@ int dsgesv(hipStream_t s, long N, long NRHS, double* A, long lda, int* ipiv, con
   hipLaunchKernelGGL(rowsum_max_kernel, dim3((unsigned)rb), dim3(kRows), 0, s, N, nchunk, part,
                      small + 2 * nr);
   ATLG_LAUNCHED();
-  getrf<float>(s, N, N, A32, N, ipiv, flags + 1, lowprec ? kGemmBf16 : kGemmNative);
+  getrf<float>(s, N, N, A32, N, ipiv, flags + 1,
+               lowprec == 1 ? kGemmBf16 : (lowprec == 2 ? kGemmBf16x6 : kGemmNative));
   ATLG_HIP(hipMemcpyAsync(hs.data(), small, bS, hipMemcpyDeviceToHost, s));
   ATLG_HIP(hipStreamSynchronize(s));
   const double anrm = as_double(hs[2 * nr]);

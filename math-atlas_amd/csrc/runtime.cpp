@@ -286,7 +286,7 @@ int atlg_sgemm(int TA, int TB, long M, long N, long K, float alpha, const float*
 }
 
 int atlg_sgemm_bf16(long M, long N, long K, float alpha, const float* A, long lda,
-                    const float* B, long ldb, float beta, float* C, long ldc,
+                    const float* B, long ldb, float beta, float* C, long ldc, int split,
                     atlg_stream_t stream) {
   if (M < 0) return -1;
   if (N < 0) return -2;
@@ -295,7 +295,7 @@ int atlg_sgemm_bf16(long M, long N, long K, float alpha, const float* A, long ld
   if (ldb < std::max(1L, K)) return -8;
   if (ldc < std::max(1L, M)) return -11;
   return guarded([&] {
-    gemm_bf16(resolve_stream(stream), M, N, K, alpha, A, lda, B, ldb, beta, C, ldc);
+    gemm_bf16(resolve_stream(stream), M, N, K, alpha, A, lda, B, ldb, beta, C, ldc, split != 0);
   });
 }
 
@@ -305,7 +305,7 @@ int atlg_sgetrf(long M, long N, float* A, long lda, int* ipiv, int* dinfo, int g
   if (N < 0) return -2;
   if (lda < std::max(1L, M)) return -4;
   if (!dinfo) return -6;
-  if (gemm_prec != kGemmNative && gemm_prec != kGemmBf16) return -7;
+  if (gemm_prec != kGemmNative && gemm_prec != kGemmBf16 && gemm_prec != kGemmBf16x6) return -7;
   return guarded(
       [&] { getrf<float>(resolve_stream(stream), M, N, A, lda, ipiv, dinfo, gemm_prec); });
 }
@@ -330,10 +330,11 @@ int atlg_dsgesv(long N, long NRHS, double* A, long lda, int* ipiv, const double*
   if (ldb < std::max(1L, N)) return -7;
   if (ldx < std::max(1L, N)) return -9;
   if (!dinfo) return -10;
+  if (lowprec < 0 || lowprec > 2) return -12;
   if (!iter) return -13;
   return guarded([&] {
     dsgesv(resolve_stream(stream), N, NRHS, A, lda, ipiv, B, ldb, X, ldx, dinfo,
-           itermax > 0 ? itermax : 30, lowprec ? 1 : 0, iter);
+           itermax > 0 ? itermax : 30, lowprec, iter);
   });
 }
 

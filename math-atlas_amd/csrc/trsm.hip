@@ -167,6 +167,171 @@ void solve(const Ctx<S>& c, long r0, long n, S alpha) {
   }
 }
 
+// ---- few right-hand sides: one sync-free cooperative launch ----------------
+//
+// op(A) = A (Left, NoTrans) with at most 4 right-hand sides.  The rows are cut
+// into 128-row blocks; workgroup w solves the blocks w, w+G, w+2G, ... in
+// solve order (top down for lower, bottom up for upper).  For its block it
+// accumulates A(block, q) x_q over the earlier blocks q as soon as each x_q
+// is published (a ready flag per block), then applies the block's inverse
+// from tri_inverse_blocks and publishes x.  The smallest unfinished block
+// always has all its inputs, so the launch cannot deadlock while every
+// workgroup is resident (cooperative launch); the waits are bounded anyway
+// and an overrun sets the error word.  Published x and the flags go through
+// agent-scope (coherence point) atomic loads and stores, as in getrf's
+// cooperative panel.
+constexpr int SV_T = 256;                  // threads: 2 per block row
+constexpr long kTrsvMin = 1024;            // shortest triangle that takes this path
+constexpr long kSvSpin = 1L << 22;         // polls (round trips) before giving up
+
+template <class S>
+__device__ inline S ld_co(const S* p);
+template <>
+__device__ inline double ld_co<double>(const double* p) {
+  const unsigned long long u = __hip_atomic_load(
+      reinterpret_cast<unsigned long long*>(const_cast<double*>(p)), __ATOMIC_RELAXED,
+      __HIP_MEMORY_SCOPE_AGENT);
+  return __longlong_as_double(static_cast<long long>(u));
+}
+template <>
+__device__ inline float ld_co<float>(const float* p) {
+  return __uint_as_float(__hip_atomic_load(reinterpret_cast<unsigned*>(const_cast<float*>(p)),
+                                           __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT));
+}
+__device__ inline void st_co(double* p, double x) {
+  __hip_atomic_store(reinterpret_cast<unsigned long long*>(p),
+                     static_cast<unsigned long long>(__double_as_longlong(x)), __ATOMIC_RELAXED,
+                     __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ inline void st_co(float* p, float x) {
+  __hip_atomic_store(reinterpret_cast<unsigned*>(p), __float_as_uint(x), __ATOMIC_RELAXED,
+                     __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ inline int ld_flag(const int* p) {
+  return __hip_atomic_load(const_cast<int*>(p), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ inline void st_flag(int* p, int v) {
+  __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+__global__ void sv_reset(int* flags, int n) {
+  for (int e = blockIdx.x * blockDim.x + threadIdx.x; e < n; e += gridDim.x * blockDim.x)
+    flags[e] = 0;
+}
+
+template <class S, int R>
+__global__ void __launch_bounds__(SV_T)
+trsv_sf_kernel(int n, int nb, int upper, S alpha, const S* __restrict__ A, long lda,
+               const S* __restrict__ Dinv, S* __restrict__ B, long ldb, int* __restrict__ flags,
+               int* __restrict__ err) {
+  __shared__ S part[2][NB][R];
+  __shared__ S xs[NB][R];
+  __shared__ int ok;
+  const int t = threadIdx.x, r = t & (NB - 1), h = t >> 7;
+  for (int li = blockIdx.x; li < nb; li += gridDim.x) {
+    const int p = upper ? nb - 1 - li : li;
+    const long r0 = (long)p * NB;
+    const int rows = (int)min((long)NB, n - r0);
+    S acc[R];
+#pragma unroll
+    for (int k = 0; k < R; k++) acc[k] = S(0);
+    for (int lj = 0; lj < li; lj++) {
+      const int q = upper ? nb - 1 - lj : lj;
+      const long c0 = (long)q * NB;
+      const int cols = (int)min((long)NB, n - c0);
+      if (t == 0) {
+        // one round trip per poll; the error word only every 256 polls
+        bool good = true;
+        long spins = 0;
+        while (ld_flag(&flags[q]) == 0) {
+          if ((++spins & 255) == 0 && (ld_flag(err) || spins > kSvSpin)) {
+            st_flag(err, 1);
+            good = false;
+            break;
+          }
+        }
+        ok = good;
+      }
+      __syncthreads();
+      if (!ok) return;  // same value in every thread
+      if (t < NB) {
+#pragma unroll
+        for (int k = 0; k < R; k++) xs[t][k] = (t < cols) ? ld_co(&B[c0 + t + (size_t)k * ldb]) : S(0);
+      }
+      __syncthreads();
+      if (r < rows) {
+        const S* a = A + (r0 + r) + (size_t)(c0 + h * 64) * lda;
+        const int cend = min(64, cols - h * 64);
+        for (int c = 0; c < cend; c++) {
+          const S v = a[(size_t)c * lda];
+#pragma unroll
+          for (int k = 0; k < R; k++) acc[k] += v * xs[h * 64 + c][k];
+        }
+      }
+      __syncthreads();
+    }
+#pragma unroll
+    for (int k = 0; k < R; k++) part[h][r][k] = acc[k];
+    __syncthreads();
+    if (t < NB) {
+#pragma unroll
+      for (int k = 0; k < R; k++)
+        xs[t][k] = (t < rows) ? alpha * B[r0 + t + (size_t)k * ldb] - part[0][t][k] - part[1][t][k]
+                              : S(0);
+    }
+    __syncthreads();
+    // x = inv(A_pp) (alpha b - acc): thread (r, h) sums 64 of the 128 columns
+    const S* D = Dinv + (size_t)p * NB * NB;
+    S y[R];
+#pragma unroll
+    for (int k = 0; k < R; k++) y[k] = S(0);
+    for (int c = 0; c < 64; c++) {
+      const int cc = h * 64 + c;
+      const S d = D[r + (size_t)cc * NB];
+#pragma unroll
+      for (int k = 0; k < R; k++) y[k] += d * xs[cc][k];
+    }
+#pragma unroll
+    for (int k = 0; k < R; k++) part[h][r][k] = y[k];
+    __syncthreads();
+    if (t < rows) {
+#pragma unroll
+      for (int k = 0; k < R; k++) st_co(&B[r0 + t + (size_t)k * ldb], part[0][t][k] + part[1][t][k]);
+    }
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+    __syncthreads();  // every store of x has completed
+    if (t == 0) st_flag(&flags[p], 1);
+  }
+}
+
+// Few-RHS Left/NoTrans solve; returns false when the shape is not handled.
+template <class S>
+bool trsv_few(hipStream_t s, bool lower, long n, long nrhs, S alpha, const S* A, long lda,
+              const S* Dinv, S* B, long ldb) {
+  if (nrhs < 1 || nrhs > 4) return false;
+  const int nb = (int)((n + NB - 1) / NB);
+  int* flags = static_cast<int*>(scratch(s, kSlotTrsv, sizeof(int) * (nb + 64)));
+  int* err = flags + nb;
+  hipLaunchKernelGGL(sv_reset, dim3((unsigned)std::min(64, (nb + 64 + 255) / 256)), dim3(256), 0,
+                     s, flags, nb + 1);
+  ATLG_LAUNCHED();
+  int dev = 0, cus = 0;
+  ATLG_HIP(hipGetDevice(&dev));
+  ATLG_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
+  int nn = (int)n, nbk = nb, up = lower ? 0 : 1;
+  int G = std::max(1, std::min(nb, cus));
+  void* args[] = {&nn, &nbk, &up, &alpha, &A, &lda, &Dinv, &B, &ldb, &flags, &err};
+  const void* fn = nullptr;
+  switch (nrhs) {
+    case 1: fn = reinterpret_cast<const void*>(&trsv_sf_kernel<S, 1>); break;
+    case 2: fn = reinterpret_cast<const void*>(&trsv_sf_kernel<S, 2>); break;
+    case 3: fn = reinterpret_cast<const void*>(&trsv_sf_kernel<S, 3>); break;
+    default: fn = reinterpret_cast<const void*>(&trsv_sf_kernel<S, 4>); break;
+  }
+  ATLG_HIP(hipLaunchCooperativeKernel(fn, dim3(G), dim3(SV_T), args, 0, s));
+  return true;
+}
+
 }  // namespace
 
 template <class S>
@@ -210,6 +375,10 @@ void trsm(hipStream_t s, int side, int uplo, int ta, int diag, long M, long N, S
     S* D = static_cast<S*>(scratch(s, kSlotTrsmInv, sizeof(S) * nblk * NB * NB));
     tri_inverse_blocks<S>(s, na, A, lda, ta, uplo, diag, D, skip);
     c.Dinv = D;
+    // a few right-hand sides on a long triangle: one sync-free launch
+    if (c.left && !c.trans && !skip && na >= kTrsvMin &&
+        trsv_few<S>(s, uplo == kLower, na, N, alpha, A, lda, D, B, ldb))
+      return;
   }
   solve(c, 0, na, alpha);
 }

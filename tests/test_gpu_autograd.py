@@ -116,5 +116,8 @@ def test_loglik_finite_differences(AG, n):
         an = float((gK * D).sum() + (gy * dy).sum())
         err = abs(fd - an) / max(abs(an), 1e-300)
         worst = max(worst, err)
+        # FD accuracy: truncation O(h^2) plus rounding, each evaluation of
+        # the log-likelihood being accurate to about u * n * |f|, divided by h
+        noise = 2.0 * 1.1e-16 * n * abs(val.item()) / (h * max(abs(an), 1e-300))
+        assert err <= 1e-4 + noise, (err, noise)
     print("N=%d: worst FD relative error over 20 directions %.2e (loglik %.6e)" % (n, worst, val.item()))
-    assert worst <= 1e-4

@@ -325,9 +325,15 @@ def test_dsposv_batched_refinement_and_fallback(G):
         if k in ill:
             assert its[k] < 0, (k, its[k])
             assert llt_ratio(mats[k], b.mat(k, host), EPS64) <= 16.0
-        else:
-            assert 0 <= its[k] <= 30, (k, its[k])
+        elif its[k] >= 0:
+            assert its[k] <= 30
             a_now = b.mat(k, host)
             assert np.array_equal(a_now, mats[k]), "converged system %d: A changed" % k
+        else:
+            # LAPACK's test ||r|| < ||x|| ||A|| eps sqrt(n) sits at FP64
+            # roundoff for the smallest orders, which may then run to the
+            # iteration cap and take the FP64 path (A holds that factor)
+            assert n < 8 and its[k] == -31, (k, n, its[k])
+            assert llt_ratio(mats[k], b.mat(k, host), EPS64) <= 16.0
     print("dsposv: iterations %s, fallbacks %d, worst backward error %.3g"
           % (np.bincount(its[its >= 0]).tolist(), int((its < 0).sum()), worst))

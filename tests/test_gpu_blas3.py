@@ -189,6 +189,30 @@ def test_trsm_l3blastst(G, side, uplo, ta, diag):
             assert r < 50.0, (m, n, alpha, r)
 
 
+@pytest.mark.parametrize("uplo", UPLOS)
+@pytest.mark.parametrize("diag", DIAGS)
+def test_trsm_few_rhs_long_triangle(G, uplo, diag):
+    # Left/NoTrans with 1..4 right-hand sides on triangles >= 1024 (the
+    # sync-free block solve), ragged last block, lda/ldb gaps, alpha != 1
+    for (m, n) in ((1024, 1), (1500, 2), (2600, 3), (1111, 4)):
+        for alpha in (1.0, -1.1):
+            lda, ldb = m + 2, m + 1
+            a = O.gegen(m, m, m * m + 513 * 7 + 90, lda=lda)
+            O.lib().orc_trddom(uplo, m, O._p(a), lda)
+            if diag == O.Unit:
+                a[np.arange(m), np.arange(m)] = np.nan   # never read
+            b0 = O.gegen(m, n, m * n + 127 * 50 + 77, lda=ldb)
+            bt = b0.copy(order="F")
+            G.cblas_dtrsm(G.ColMajor, O.Left, uplo, O.NoTrans, diag, m, n, alpha, a, lda, bt, ldb)
+            br = b0.copy(order="F")
+            O.lib().orc_reftrsm(O.Left, uplo, O.NoTrans, diag, m, n, alpha, O._p(a), lda,
+                                O._p(br), ldb)
+            assert O.gechkgap(bt, m) == 0
+            r = O.lib().orc_trsm_resid(O.Left, uplo, diag, m, n, alpha, O._p(a), lda, O._p(b0),
+                                       ldb, O._p(bt), ldb, O._p(br), ldb)
+            assert r < 50.0, (m, n, alpha, r)
+
+
 def test_trsm_rowmajor_and_device_api(G):
     import torch
     m, n = 300, 200

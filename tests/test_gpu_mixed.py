@@ -4,7 +4,9 @@
   within K * eps32 * (|A||B|)_ij (ATLAS's gemmtst element bound, in FP32);
 * atlg_sgemm_bf16 against an FP64 product of the operands rounded to
   bfloat16 in numpy (round to nearest even): BF16 x BF16 products are exact
-  in FP32, so the same K * eps32 bound holds;
+  in FP32, so the same K * eps32 bound holds; the split form (hi + mid + lo
+  operands, six products) against the same six products in FP64, and at
+  FP32 accuracy against the FP32 product;
 * atlg_sgetrf: lutst residual ||P A - L U||_1 / (||A||_1 eps n) <= 100 with
   eps of the arithmetic (FP32; BF16 updates: the bfloat16 unit roundoff),
   and pivots identical to the oracle's on matrices with well-separated
@@ -72,27 +74,47 @@ def test_sgemm(G, ta, tb, m, n, k):
     assert np.all(np.abs(got - ref) <= bound + 1e-30)
 
 
+@pytest.mark.parametrize("split", [0, 1])
 @pytest.mark.parametrize("m,n,k", [(300, 200, 100), (128, 256, 64), (1031, 517, 777)])
-def test_sgemm_bf16(G, m, n, k):
+def test_sgemm_bf16(G, m, n, k, split):
     import torch
     rng = np.random.default_rng(m + n + k)
     a = rng.uniform(-1, 1, (m, k)).astype(np.float32)
     b = rng.uniform(-1, 1, (k, n)).astype(np.float32)
     c = rng.uniform(-1, 1, (m, n)).astype(np.float32)
     da, db, dc = dev(a), dev(b), dev(c)
-    G.sgemm_bf16(m, n, k, -1.0, da, m, db, k, 1.0, dc, m)
+    G.sgemm_bf16(m, n, k, -1.0, da, m, db, k, 1.0, dc, m, split)
     torch.cuda.synchronize()
     got = host(dc, m, n, m)
-    a16, b16 = to_bf16(a).astype(np.float64), to_bf16(b).astype(np.float64)
-    ref = c.astype(np.float64) - a16 @ b16
-    bound = (np.abs(a16) @ np.abs(b16) + np.abs(c)) * (k + 2) * EPS32
+    a_hi, b_hi = to_bf16(a), to_bf16(b)
+    a16, b16 = a_hi.astype(np.float64), b_hi.astype(np.float64)
+    if split:
+        a_mid = to_bf16(a - a_hi)
+        b_mid = to_bf16(b - b_hi)
+        a_lo = to_bf16(a - a_hi - a_mid).astype(np.float64)
+        b_lo = to_bf16(b - b_hi - b_mid).astype(np.float64)
+        a_mid, b_mid = a_mid.astype(np.float64), b_mid.astype(np.float64)
+        terms = [(a16, b16), (a16, b_mid), (a_mid, b16), (a_mid, b_mid), (a16, b_lo), (a_lo, b16)]
+        prod = sum(x @ y for x, y in terms)
+        mag = sum(np.abs(x) @ np.abs(y) for x, y in terms)
+    else:
+        prod = a16 @ b16
+        mag = np.abs(a16) @ np.abs(b16)
+    ref = c.astype(np.float64) - prod
+    bound = (mag + np.abs(c)) * (6 * k + 2) * EPS32
     err = np.abs(got - ref)
-    print("bf16 gemm %dx%dx%d: max err %.3g, max err/bound %.3g"
-          % (m, n, k, err.max(), (err / (bound + 1e-30)).max()))
+    print("bf16 gemm split=%d %dx%dx%d: max err %.3g, max err/bound %.3g"
+          % (split, m, n, k, err.max(), (err / (bound + 1e-30)).max()))
     assert np.all(err <= bound + 1e-30)
-    # and the BF16 rounding really happened: an FP32 product differs
     ref32 = c.astype(np.float64) - a.astype(np.float64) @ b.astype(np.float64)
-    assert np.abs(got - ref32).max() > 1e3 * np.abs(got - ref).max()
+    err32 = np.abs(got - ref32).max()
+    if split:
+        # FP32 accuracy: within the FP32 GEMM element bound of the true product
+        b32 = (np.abs(a.astype(np.float64)) @ np.abs(b.astype(np.float64)) + np.abs(c)) * (k + 2) * EPS32
+        assert np.all(np.abs(got - ref32) <= 2 * b32)
+    else:
+        # the BF16 rounding really happened: an FP32 product differs
+        assert err32 > 1e3 * err.max()
 
 
 def lu_ratio(a0, lu, ipiv, eps):
@@ -108,7 +130,7 @@ def lu_ratio(a0, lu, ipiv, eps):
     return np.abs(pa - L @ U).sum(axis=0).max() / (np.abs(a0).sum(axis=0).max() * eps * mn)
 
 
-@pytest.mark.parametrize("prec", [0, 1])
+@pytest.mark.parametrize("prec", [0, 1, 2])
 @pytest.mark.parametrize("m,n", [(700, 700), (1536, 1536), (2000, 901), (901, 1300)])
 def test_sgetrf(G, prec, m, n):
     import torch
@@ -121,7 +143,7 @@ def test_sgetrf(G, prec, m, n):
     torch.cuda.synchronize()
     assert int(info.item()) == 0
     lu = host(da, m, n, m)
-    r = lu_ratio(a.astype(np.float64), lu, ipiv.cpu().numpy(), EPS32 if prec == 0 else U_BF16)
+    r = lu_ratio(a.astype(np.float64), lu, ipiv.cpu().numpy(), U_BF16 if prec == 1 else EPS32)
     print("sgetrf prec=%d %dx%d: ratio %.3g" % (prec, m, n, r))
     assert r <= 100.0
 
@@ -182,7 +204,7 @@ def backward_error(a, x, b):
                  + np.abs(b).max(axis=0))).max()
 
 
-@pytest.mark.parametrize("lowprec", [0, 1])
+@pytest.mark.parametrize("lowprec", [0, 1, 2])
 @pytest.mark.parametrize("n,nrhs,kappa", [(1024, 1, 1e2), (1536, 1, 1e6), (777, 3, 1e2),
                                           (1024, 1, 1e10), (600, 6, 1e6)])
 def test_dsgesv(G, lowprec, n, nrhs, kappa):
@@ -209,7 +231,7 @@ def test_dsgesv(G, lowprec, n, nrhs, kappa):
         assert it in (-2, -3, -31)
         lu = host(da, n, n, n)
         assert lu_ratio(a, lu, ipiv.cpu().numpy(), np.finfo(np.float64).eps / 2) <= 100.0
-    if kappa <= 1e6 and lowprec == 0:
-        assert it >= 0  # FP32 refinement converges for kappa * u32 << 1
-    if kappa >= 1e10 and lowprec == 0:
+    if kappa <= 1e6 and lowprec in (0, 2):
+        assert it >= 0  # FP32-accurate factor: refinement converges for kappa * u32 < 1
+    if kappa >= 1e10 and lowprec in (0, 2):
         assert it < 0
