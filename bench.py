@@ -215,9 +215,14 @@ def bench_ir(torch, G, n):
         anrm = float(Acm.abs().sum(dim=0).max())  # ||A||_inf: column sums of A^T
         tag = "%.0e" % kappa
 
+        ax = torch.empty_like(x)
+
         def berr():
-            r = b - torch.mv(Acm.t(), x)  # A x with A^T stored row-major
-            return float(r.abs().max() / (anrm * x.abs().max() + b.abs().max()))
+            # A x through this library's DGEMM on the column-major buffer of A
+            G.dgemm(G.NoTrans, G.NoTrans, n, 1, n, 1.0, Acm, n, x, n, 0.0, ax, n)
+            r = b - ax
+            rn, xn, bn = float(r.abs().max()), float(x.abs().max()), float(b.abs().max())
+            return rn / (anrm * xn + bn), rn, xn
 
         def run64():
             A.copy_(Acm)
@@ -236,13 +241,14 @@ def bench_ir(torch, G, n):
             out["ir_setup_s"] = round(_t.time() - t0, 1)
             ms64 = timed(torch, run64, 1) - ms_copy
             out["ir_dgesv_fp64_ms"] = ms64
-            out["ir_dgesv_fp64_berr"] = berr()
+            out["ir_dgesv_fp64_berr"], out["ir_dgesv_fp64_rnorm"], out["ir_dgesv_fp64_xnorm"] = berr()
+            out["ir_anorm"] = anrm
             out["ir_dgetrf_fp64_tflops"] = (2.0 * n ** 3 / 3) / ms64 / 1e9
         for lp, name in ((0, "fp32"), (1, "bf16"), (2, "bf16x6")):
             ms = timed(torch, lambda: runmix(lp), 1) - ms_copy
             out["ir_k%s_%s_ms" % (tag, name)] = ms
             out["ir_k%s_%s_iter" % (tag, name)] = its[lp]
-            out["ir_k%s_%s_berr" % (tag, name)] = berr()
+            out["ir_k%s_%s_berr" % (tag, name)] = berr()[0]
             out["ir_k%s_%s_speedup" % (tag, name)] = out["ir_dgesv_fp64_ms"] / ms
         del Acm, A
     del U, V

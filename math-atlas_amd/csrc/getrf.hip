@@ -322,6 +322,8 @@ panel_coop(int M, int w, int off, S* __restrict__ P, long lda, int* __restrict__
            int* __restrict__ dinfo, CoopBuf<S>* __restrict__ cb, int* __restrict__ list) {
   __shared__ S prow[PW], jrow[PW];
   __shared__ S crows[CP_MAXG * PW];
+  __shared__ S stage_c[PW], stage_j[PW];  // this workgroup's candidate row / row j
+  __shared__ int has_c, has_j;
   __shared__ double sk[CP_THREADS / 64];
   __shared__ int si[CP_THREADS / 64];
   __shared__ int winner, verdict;
@@ -370,38 +372,59 @@ panel_coop(int M, int w, int off, S* __restrict__ P, long lda, int* __restrict__
       if (t == 0) winner = idx;
       __syncthreads();
     } else {
-    bool wrote = false;
+    // Publish: the owners copy the candidate row and row j to LDS, then one
+    // lane per element stores them (an agent-scope atomic store waits for
+    // the stores before it, so one lane storing a whole row would pay a
+    // round trip per element).
+    if (t == 0) {
+      has_c = 0;
+      has_j = 0;
+    }
+    __syncthreads();
 #pragma unroll
     for (int q = 0; q < RPT; q++) {
       const int i = g + q * GT;
       if (i == idx) {
 #pragma unroll
-        for (int c = 0; c < PW; c++)
-          if (c < w) st_agent(&cb->crow[par][blockIdx.x][c], v[q][c]);
-        wrote = true;
+        for (int c = 0; c < PW; c++) stage_c[c] = v[q][c];
+        has_c = 1;
       }
       if (i == j) {
 #pragma unroll
-        for (int c = 0; c < PW; c++)
-          if (c < w) st_agent(&cb->jrow[par][c], v[q][c]);
-        wrote = true;
+        for (int c = 0; c < PW; c++) stage_j[c] = v[q][c];
+        has_j = 1;
       }
     }
-    if (t == 0) {
+    __syncthreads();
+    if (t < 64) {
+      const int c = t & (PW - 1);
+      if (t < PW) {
+        if (has_c && c < w) st_agent(&cb->crow[par][blockIdx.x][c], stage_c[c]);
+      } else if (has_j && c < w) {
+        st_agent(&cb->jrow[par][c], stage_j[c]);
+      }
+    }
+    if (t == 64) {
       st_agent(&cb->cand[par][blockIdx.x].key, key);
       st_agent(&cb->cand[par][blockIdx.x].idx, idx);
-      wrote = true;
     }
-    (void)wrote;
     if (!grid_sync(cb, (unsigned)G * (j + 1), &verdict)) {
       ok = false;
       break;
     }
     // one round trip: every workgroup reads all G candidates, all G
     // candidate rows and row j, then picks the winner locally
-    for (int e = t; e < G * PW; e += CP_THREADS) {
-      const int b = e / PW, c = e - b * PW;
-      crows[e] = (c < w) ? ld_agent(&cb->crow[par][b][c]) : S(0);
+    {
+      constexpr int NU = CP_MAXG * PW / CP_THREADS;  // loads per thread, all in flight
+      S tmp[NU];
+#pragma unroll
+      for (int u = 0; u < NU; u++) {
+        const int e = t + u * CP_THREADS, bq = e / PW, c = e - bq * PW;
+        tmp[u] = (e < G * PW && c < w) ? ld_agent(&cb->crow[par][bq][c]) : S(0);
+      }
+#pragma unroll
+      for (int u = 0; u < NU; u++)
+        if (t + u * CP_THREADS < G * PW) crows[t + u * CP_THREADS] = tmp[u];
     }
     if (t < PW) jrow[t] = (t < w) ? ld_agent(&cb->jrow[par][t]) : S(0);
     if (t < 64) {

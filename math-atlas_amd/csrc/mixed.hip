@@ -88,13 +88,14 @@ gemv_partial_kernel(long N, int nrhs, const double* __restrict__ A, long lda,
     if (k < nrhs) partial[((size_t)c * 4 + k) * N + i] = acc[k];
 }
 
-// R = B - sum_c partial[c];  W32 = float(R);  per column k: atomic max of
-// |r| and |x| into nrm[2k], nrm[2k+1] (non-negative doubles compare as
+// R = B - sum_c partial[c];  W32[pos[i]] = float(R[i]) (the row
+// interchanges of the factor applied on the way);  per column k: atomic max
+// of |r| and |x| into nrm[2k], nrm[2k+1] (non-negative doubles compare as
 // their bit patterns).
 __global__ void __launch_bounds__(kRows)
 residual_finish_kernel(long N, int nrhs, int nchunk, const double* __restrict__ partial,
                        const double* __restrict__ B, long ldb, const double* __restrict__ X,
-                       long ldx, float* __restrict__ W32, long ldw,
+                       long ldx, float* __restrict__ W32, long ldw, const int* __restrict__ pos,
                        unsigned long long* __restrict__ nrm) {
   const long i = (long)blockIdx.x * kRows + threadIdx.x;
   for (int k = 0; k < nrhs; k++) {
@@ -104,7 +105,7 @@ residual_finish_kernel(long N, int nrhs, int nchunk, const double* __restrict__ 
       for (int c = 0; c < nchunk; c++) s += partial[((size_t)c * 4 + k) * N + i];
       r = B[i + (size_t)k * ldb] - s;
       x = X[i + (size_t)k * ldx];
-      W32[i + (size_t)k * ldw] = (float)r;
+      W32[pos[i] + (size_t)k * ldw] = (float)r;
     }
     double rm = fabs(r), xm = fabs(x);
     for (int o = 32; o > 0; o >>= 1) {
@@ -130,13 +131,14 @@ rowsum_max_kernel(long N, int nchunk, const double* __restrict__ partial,
   if ((threadIdx.x & 63) == 0) atomicMax(anrm, (unsigned long long)__double_as_longlong(s));
 }
 
+// W[pos[i]] = float(B[i]) per column
 __global__ void d2s_kernel(long N, int nrhs, const double* __restrict__ B, long ldb,
-                           float* __restrict__ W, long ldw) {
+                           float* __restrict__ W, long ldw, const int* __restrict__ pos) {
   const long total = N * nrhs;
   for (long e = blockIdx.x * (long)blockDim.x + threadIdx.x; e < total;
        e += (long)gridDim.x * blockDim.x) {
     const long i = e % N, k = e / N;
-    W[i + k * ldw] = (float)B[i + k * ldb];
+    W[pos[i] + k * ldw] = (float)B[i + k * ldb];
   }
 }
 
@@ -184,12 +186,19 @@ int dsgesv(hipStream_t s, long N, long NRHS, double* A, long lda, int* ipiv, con
   const size_t bW = ((size_t)N * NRHS * sizeof(float) + 255) & ~size_t(255);
   const size_t bP = ((size_t)nchunk * 4 * N * sizeof(double) + 255) & ~size_t(255);
   const int nsmall = 2 * nr + 2;
-  const size_t bS = sizeof(unsigned long long) * nsmall;
-  char* base = static_cast<char*>(scratch(s, kSlotMixed, bA32 + bW + bP + bS));
+  const size_t bS = (sizeof(unsigned long long) * nsmall + 255) & ~size_t(255);
+  const size_t bPos = (sizeof(int) * N + 255) & ~size_t(255);
+  const long nblk = (N + kTrsmNB - 1) / kTrsmNB;
+  const size_t bInv = sizeof(float) * nblk * kTrsmNB * kTrsmNB;
+  char* base = static_cast<char*>(
+      scratch(s, kSlotMixed, bA32 + bW + bP + bS + bPos + 2 * bInv));
   float* A32 = reinterpret_cast<float*>(base);
   float* W32 = reinterpret_cast<float*>(base + bA32);
   double* part = reinterpret_cast<double*>(base + bA32 + bW);
   unsigned long long* small = reinterpret_cast<unsigned long long*>(base + bA32 + bW + bP);
+  int* dpos = reinterpret_cast<int*>(base + bA32 + bW + bP + bS);
+  float* invL = reinterpret_cast<float*>(base + bA32 + bW + bP + bS + bPos);
+  float* invU = invL + (size_t)nblk * kTrsmNB * kTrsmNB;
   int* flags = reinterpret_cast<int*>(small + 2 * nr + 1);  // [0] overflow, [1] FP32 info
   std::vector<unsigned long long> hs(nsmall);
   const unsigned gs = (unsigned)std::min(64, (nsmall + 255) / 256);
@@ -205,7 +214,10 @@ int dsgesv(hipStream_t s, long N, long NRHS, double* A, long lda, int* ipiv, con
   ATLG_LAUNCHED();
   getrf<float>(s, N, N, A32, N, ipiv, flags + 1,
                lowprec == 1 ? kGemmBf16 : (lowprec == 2 ? kGemmBf16x6 : kGemmNative));
-  ATLG_HIP(hipMemcpyAsync(hs.data(), small, bS, hipMemcpyDeviceToHost, s));
+  std::vector<int> hpiv(N);
+  ATLG_HIP(hipMemcpyAsync(hs.data(), small, sizeof(unsigned long long) * nsmall,
+                          hipMemcpyDeviceToHost, s));
+  ATLG_HIP(hipMemcpyAsync(hpiv.data(), ipiv, sizeof(int) * N, hipMemcpyDeviceToHost, s));
   ATLG_HIP(hipStreamSynchronize(s));
   const double anrm = as_double(hs[2 * nr]);
   int hflags[2];
@@ -219,11 +231,31 @@ int dsgesv(hipStream_t s, long N, long NRHS, double* A, long lda, int* ipiv, con
   } else if (hflags[1] != 0) {
     iter = -3;
   } else {
+    // The factor's row interchanges composed once: row i of P b is row
+    // perm[i] of b, so b's row i lands at pos[i] = perm^-1[i].  They are
+    // applied inside the conversions below, and the triangles' diagonal-
+    // block inverses are computed once for all the solves.
+    {
+      std::vector<int> perm(N), pos(N);
+      for (long i = 0; i < N; i++) perm[i] = (int)i;
+      for (long i = 0; i < N; i++) std::swap(perm[i], perm[hpiv[i]]);
+      for (long i = 0; i < N; i++) pos[perm[i]] = (int)i;
+      ATLG_HIP(hipMemcpyAsync(dpos, pos.data(), sizeof(int) * N, hipMemcpyHostToDevice, s));
+      ATLG_HIP(hipStreamSynchronize(s));  // pos lives on this stack frame
+    }
+    tri_inverse_blocks<float>(s, N, A32, N, kNoTrans, kLower, kUnit, invL);
+    tri_inverse_blocks<float>(s, N, A32, N, kNoTrans, kUpper, kNonUnit, invU);
+    auto solve32 = [&]() {
+      trsm<float>(s, kLeft, kLower, kNoTrans, kUnit, N, NRHS, 1.0f, A32, N, W32, ldw, nullptr,
+                  invL, 0);
+      trsm<float>(s, kLeft, kUpper, kNoTrans, kNonUnit, N, NRHS, 1.0f, A32, N, W32, ldw,
+                  nullptr, invU, 0);
+    };
     // x = solve32(b)
     hipLaunchKernelGGL(d2s_kernel, dim3(grid1(N * NRHS)), dim3(256), 0, s, N, nr, B, ldb, W32,
-                       ldw);
+                       ldw, dpos);
     ATLG_LAUNCHED();
-    getrs<float>(s, kNoTrans, N, NRHS, A32, N, ipiv, W32, ldw);
+    solve32();
     hipLaunchKernelGGL(s2d_kernel, dim3(grid1(N * NRHS)), dim3(256), 0, s, N, nr, W32, ldw, X,
                        ldx, 0);
     ATLG_LAUNCHED();
@@ -239,7 +271,7 @@ int dsgesv(hipStream_t s, long N, long NRHS, double* A, long lda, int* ipiv, con
         ATLG_LAUNCHED();
         hipLaunchKernelGGL(residual_finish_kernel, dim3((unsigned)rb), dim3(kRows), 0, s, N, kk,
                            nchunk, part, B + (size_t)k0 * ldb, ldb, X + (size_t)k0 * ldx, ldx,
-                           W32 + (size_t)k0 * ldw, ldw, small + 2 * k0);
+                           W32 + (size_t)k0 * ldw, ldw, dpos, small + 2 * k0);
         ATLG_LAUNCHED();
       }
       ATLG_HIP(hipMemcpyAsync(hs.data(), small, sizeof(unsigned long long) * 2 * nr,
@@ -255,7 +287,7 @@ int dsgesv(hipStream_t s, long N, long NRHS, double* A, long lda, int* ipiv, con
       }
       if (k == itermax) break;
       // x += solve32(r)
-      getrs<float>(s, kNoTrans, N, NRHS, A32, N, ipiv, W32, ldw);
+      solve32();
       hipLaunchKernelGGL(s2d_kernel, dim3(grid1(N * NRHS)), dim3(256), 0, s, N, nr, W32, ldw, X,
                          ldx, 1);
       ATLG_LAUNCHED();

@@ -239,6 +239,15 @@ trsv_sf_kernel(int n, int nb, int upper, S alpha, const S* __restrict__ A, long 
       const int q = upper ? nb - 1 - lj : lj;
       const long c0 = (long)q * NB;
       const int cols = (int)min((long)NB, n - c0);
+      // this thread's 64 entries of A(block p, block q): issued before the
+      // wait, they do not depend on x_q
+      S va[64];
+      {
+        const int cend = (r < rows) ? min(64, cols - h * 64) : 0;
+        const S* a = A + (r0 + r) + (size_t)(c0 + h * 64) * lda;
+#pragma unroll
+        for (int c = 0; c < 64; c++) va[c] = (c < cend) ? a[(size_t)c * lda] : S(0);
+      }
       if (t == 0) {
         // one round trip per poll; the error word only every 256 polls
         bool good = true;
@@ -259,15 +268,10 @@ trsv_sf_kernel(int n, int nb, int upper, S alpha, const S* __restrict__ A, long 
         for (int k = 0; k < R; k++) xs[t][k] = (t < cols) ? ld_co(&B[c0 + t + (size_t)k * ldb]) : S(0);
       }
       __syncthreads();
-      if (r < rows) {
-        const S* a = A + (r0 + r) + (size_t)(c0 + h * 64) * lda;
-        const int cend = min(64, cols - h * 64);
-        for (int c = 0; c < cend; c++) {
-          const S v = a[(size_t)c * lda];
 #pragma unroll
-          for (int k = 0; k < R; k++) acc[k] += v * xs[h * 64 + c][k];
-        }
-      }
+      for (int c = 0; c < 64; c++)
+#pragma unroll
+        for (int k = 0; k < R; k++) acc[k] += va[c] * xs[h * 64 + c][k];
       __syncthreads();
     }
 #pragma unroll
@@ -281,15 +285,19 @@ trsv_sf_kernel(int n, int nb, int upper, S alpha, const S* __restrict__ A, long 
     }
     __syncthreads();
     // x = inv(A_pp) (alpha b - acc): thread (r, h) sums 64 of the 128 columns
-    const S* D = Dinv + (size_t)p * NB * NB;
+    const S* D = Dinv + (size_t)p * NB * NB + r + (size_t)h * 64 * NB;
     S y[R];
 #pragma unroll
     for (int k = 0; k < R; k++) y[k] = S(0);
-    for (int c = 0; c < 64; c++) {
-      const int cc = h * 64 + c;
-      const S d = D[r + (size_t)cc * NB];
 #pragma unroll
-      for (int k = 0; k < R; k++) y[k] += d * xs[cc][k];
+    for (int c0 = 0; c0 < 64; c0 += 16) {
+      S d[16];
+#pragma unroll
+      for (int c = 0; c < 16; c++) d[c] = D[(size_t)(c0 + c) * NB];
+#pragma unroll
+      for (int c = 0; c < 16; c++)
+#pragma unroll
+        for (int k = 0; k < R; k++) y[k] += d[c] * xs[h * 64 + c0 + c][k];
     }
 #pragma unroll
     for (int k = 0; k < R; k++) part[h][r][k] = y[k];
@@ -370,6 +378,9 @@ void trsm(hipStream_t s, int side, int uplo, int ta, int diag, long M, long N, S
   c.dtrans = dinv ? (dinv_trans ? kTrans : kNoTrans) : kNoTrans;
   if (dinv) {
     c.Dinv = dinv;
+    if (c.left && !c.trans && !dinv_trans && !skip && na >= kTrsvMin &&
+        trsv_few<S>(s, uplo == kLower, na, N, alpha, A, lda, dinv, B, ldb))
+      return;
   } else {
     const long nblk = (na + NB - 1) / NB;
     S* D = static_cast<S*>(scratch(s, kSlotTrsmInv, sizeof(S) * nblk * NB * NB));
