@@ -192,9 +192,10 @@ def haar(torch, n, g):
 def bench_ir(torch, G, n):
     """Mixed-precision dgesv at order n, one RHS b iid normal(0,1),
     A = U diag(s) V^T with U, V Haar orthogonal and s_i = kappa^(-i/(n-1)),
-    kappa in {1e2, 1e6, 1e10}: plain FP64 dgesv (getrf + getrs) against the
-    FP32-LU and BF16-LU refinement (atlg_dsgesv), with each solution's
-    normwise backward error ||b - A x||_inf / (||A||_inf ||x||_inf + ||b||_inf)."""
+    kappa in {1e2, 1e6, 1e10}: plain FP64 dgesv (getrf + getrs) against
+    atlg_dsgesv with FP32, BF16 and split-BF16 factors, each with classical
+    refinement and with GMRES-IR, and each solution's normwise backward
+    error ||b - A x||_inf / (||A||_inf ||x||_inf + ||b||_inf)."""
     import time as _t
     g = torch.Generator(device="cuda")
     g.manual_seed(32768)
@@ -232,9 +233,9 @@ def bench_ir(torch, G, n):
 
         its = {}
 
-        def runmix(lp):
+        def runmix(lp, rf):
             A.copy_(Acm)
-            its[lp] = G.dsgesv(n, 1, A, n, ipiv, b, n, x, n, info, 30, lp)
+            its[lp] = G.dsgesv(n, 1, A, n, ipiv, b, n, x, n, info, 30, lp, refine=rf, inner=True)
 
         ms_copy = timed(torch, lambda: A.copy_(Acm), 2)
         if kappa == 1e2:
@@ -245,11 +246,13 @@ def bench_ir(torch, G, n):
             out["ir_anorm"] = anrm
             out["ir_dgetrf_fp64_tflops"] = (2.0 * n ** 3 / 3) / ms64 / 1e9
         for lp, name in ((0, "fp32"), (1, "bf16"), (2, "bf16x6")):
-            ms = timed(torch, lambda: runmix(lp), 1) - ms_copy
-            out["ir_k%s_%s_ms" % (tag, name)] = ms
-            out["ir_k%s_%s_iter" % (tag, name)] = its[lp]
-            out["ir_k%s_%s_berr" % (tag, name)] = berr()[0]
-            out["ir_k%s_%s_speedup" % (tag, name)] = out["ir_dgesv_fp64_ms"] / ms
+            for rf, rname in ((0, "ir"), (1, "gmres")):
+                ms = timed(torch, lambda: runmix(lp, rf), 1) - ms_copy
+                key = "ir_k%s_%s_%s" % (tag, name, rname)
+                out[key + "_ms"] = ms
+                out[key + "_iter"], out[key + "_inner"] = its[lp]
+                out[key + "_berr"] = berr()[0]
+                out[key + "_speedup"] = out["ir_dgesv_fp64_ms"] / ms
         del Acm, A
     del U, V
     return out

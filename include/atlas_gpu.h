@@ -110,7 +110,14 @@ int atlg_dlogdet_chol_adjoint(long N, const double *L, long ldl,
  *   corrections used, or -2 (A overflows FP32), -3 (FP32 factor singular),
  *   -(itermax+1) (no convergence); when negative, A holds the FP64 LU
  *   factors and X the FP64 dgetrs solution.  ipiv holds the pivots of
- *   whichever factor was used.  Synchronizes the stream. */
+ *   whichever factor was used.  refine 0: classical refinement (LAPACK),
+ *   x += solve(r); refine 1: GMRES-IR (Carson and Higham 2018), each
+ *   correction solves A d = r by FP64 GMRES left-preconditioned with the
+ *   low-precision factor (at most 30 iterations, tolerance 1e-6), which converges for
+ *   condition numbers well beyond 1/u of the factor; it stops early, with
+ *   ITER = -(itermax+1) and the FP64 fallback, when two corrections in a
+ *   row fail to halve the residual; *inner (host, may be NULL) receives
+ *   the GMRES iterations.  Synchronizes the stream. */
 int atlg_sgemm(int TA, int TB, long M, long N, long K, float alpha,
                const float *A, long lda, const float *B, long ldb, float beta,
                float *C, long ldc, atlg_stream_t stream);
@@ -123,7 +130,8 @@ int atlg_sgetrs(int Trans, long N, long NRHS, const float *A, long lda,
                 const int *ipiv, float *B, long ldb, atlg_stream_t stream);
 int atlg_dsgesv(long N, long NRHS, double *A, long lda, int *ipiv,
                 const double *B, long ldb, double *X, long ldx, int *dinfo,
-                int itermax, int lowprec, int *iter, atlg_stream_t stream);
+                int itermax, int lowprec, int refine, int *iter, int *inner,
+                atlg_stream_t stream);
 
 /* ---- Batched SPD solvers (no ATLAS counterpart; per system the same
  * lower Cholesky as ATL_dpotrf and the two solves of ATL_dpotrs,

@@ -80,7 +80,7 @@ _SIGS = {
                              _i, _v]),
     "atlg_sgetrf": (_i, [_l, _l, _v, _l, _v, _v, _i, _v]),
     "atlg_sgetrs": (_i, [_i, _l, _l, _v, _l, _v, _v, _l, _v]),
-    "atlg_dsgesv": (_i, [_l, _l, _v, _l, _v, _v, _l, _v, _l, _v, _i, _i, _ip, _v]),
+    "atlg_dsgesv": (_i, [_l, _l, _v, _l, _v, _v, _l, _v, _l, _v, _i, _i, _i, _ip, _ip, _v]),
     "atlg_dpotrf_batched": (_i, [_i, _ip, _v, _ip, _v, _v]),
     "atlg_dpotrs_batched": (_i, [_i, _ip, _v, _ip, _v, _ip, _i, _v]),
     "atlg_dposv_batched": (_i, [_i, _ip, _v, _ip, _v, _ip, _i, _v, _v]),
@@ -410,10 +410,11 @@ def sgetrs(trans, n, nrhs, a, lda, ipiv, b, ldb, stream=None):
                              _stream(stream)), "atlg_sgetrs")
 
 
-def dsgesv(n, nrhs, a, lda, ipiv, b, ldb, x, ldx, info, itermax=30, lowprec=0, stream=None):
-    """Returns ITER (LAPACK dsgesv meaning)."""
-    it = ctypes.c_int(0)
+def dsgesv(n, nrhs, a, lda, ipiv, b, ldb, x, ldx, info, itermax=30, lowprec=0, refine=0,
+           stream=None, inner=False):
+    """Returns ITER (LAPACK dsgesv meaning); with inner=True, (ITER, GMRES iterations)."""
+    it, inn = ctypes.c_int(0), ctypes.c_int(0)
     _check(lib().atlg_dsgesv(n, nrhs, _ptr(a), lda, _ptr(ipiv), _ptr(b), ldb, _ptr(x), ldx,
-                             _ptr(info), itermax, lowprec, ctypes.byref(it), _stream(stream)),
-           "atlg_dsgesv")
-    return it.value
+                             _ptr(info), itermax, lowprec, refine, ctypes.byref(it),
+                             ctypes.byref(inn), _stream(stream)), "atlg_dsgesv")
+    return (it.value, inn.value) if inner else it.value

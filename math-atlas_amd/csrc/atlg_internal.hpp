@@ -88,10 +88,12 @@ void* scratch(hipStream_t s, Slot slot, size_t bytes);
 // only.  `skip` (device int, may be null): when *skip != 0 the launch does
 // nothing.  C may alias B when M <= 128 and beta == 0, or alias A when
 // N <= 128 and beta == 0 (used by TRSM's in-place diagonal apply).
+// max_wg > 0 caps the grid (a multiple of 8 workgroups loop over the
+// tiles), leaving room on the device for concurrent work.
 template <class S>
 void gemm(hipStream_t s, int TA, int TB, long M, long N, long K, S alpha, const S* A, long lda,
           const S* B, long ldb, S beta, S* C, long ldc, int mode = 0,
-          const int* skip = nullptr);
+          const int* skip = nullptr, int max_wg = 0);
 // C <- beta*C on the M x N (or triangular per mode) part; beta==0 zeroes.
 template <class S>
 void scale(hipStream_t s, long M, long N, S beta, S* C, long ldc, int mode = 0,
@@ -176,11 +178,13 @@ void dsposv_batched(hipStream_t s, int nsys, const int* n, double* const* A, con
 void gemm_bf16(hipStream_t s, long M, long N, long K, float alpha, const float* A, long lda,
                const float* B, long ldb, float beta, float* C, long ldc, bool split = false);
 
-// Mixed-precision dgesv (LAPACK dsgesv semantics): FP32 LU (lowprec != 0:
-// BF16 trailing updates), FP64 residual refinement, FP64 fallback.  *iter_out
-// receives ITER.  Synchronizes the stream.
+// Mixed-precision dgesv (LAPACK dsgesv semantics): FP32 LU (lowprec 1: BF16
+// trailing updates, 2: split BF16), FP64 residual refinement (refine 0:
+// classical, 1: GMRES-IR), FP64 fallback.  *iter_out receives ITER,
+// *inner_out (if not null) the GMRES iterations.  Synchronizes the stream.
 int dsgesv(hipStream_t s, long N, long NRHS, double* A, long lda, int* ipiv, const double* B,
-           long ldb, double* X, long ldx, int* dinfo, int itermax, int lowprec, int* iter_out);
+           long ldb, double* X, long ldx, int* dinfo, int itermax, int lowprec, int refine,
+           int* iter_out, int* inner_out);
 
 // zero an int on device (kernel, graph-capture safe)
 void set_int(hipStream_t s, int* p, int v);

@@ -162,14 +162,19 @@ gemm_mfma(int M, int N, int K, S alpha, const S* A, long lda, const S* B, long l
   typedef Mfma<S> MF;
   if (skip && *skip) return;
   const int tilesM = (M + BM - 1) / BM, tilesN = (N + BN - 1) / BN;
-  int tm, tn;
-  tile_of(blockIdx.x, gridDim.x, tilesM, tilesN, tm, tn);
-  const int m0 = tm * BM, n0 = tn * BN;
-  if (mode == 1 && n0 >= m0 + BM) return;  // tile strictly above the diagonal
-  if (mode == 2 && m0 >= n0 + BN) return;  // tile strictly below the diagonal
+  const int ntiles = tilesM * tilesN;
 
   __shared__ __attribute__((aligned(16))) S As[2][BK * LDA_S];
   __shared__ __attribute__((aligned(16))) S Bs[2][BK * LDB_S];
+
+  // One tile per workgroup, or (a capped grid, its size a multiple of 8 so
+  // that a workgroup keeps its XCD) a strided loop over the tiles.
+  for (int bid = blockIdx.x; bid < ntiles; bid += gridDim.x) {
+  int tm, tn;
+  tile_of(bid, ntiles, tilesM, tilesN, tm, tn);
+  const int m0 = tm * BM, n0 = tn * BN;
+  if (mode == 1 && n0 >= m0 + BM) continue;  // tile strictly above the diagonal
+  if (mode == 2 && m0 >= n0 + BN) continue;  // tile strictly below the diagonal
 
   const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
   const int wm0 = (wave >> 1) * WM, wn0 = (wave & 1) * WN;
@@ -239,6 +244,8 @@ gemm_mfma(int M, int N, int K, S alpha, const S* A, long lda, const S* B, long l
         const S v = alpha * acc[i][j][r];
         *c = (beta == S(0)) ? v : v + beta * (*c);
       }
+  __syncthreads();  // the next tile's first loads overwrite As/Bs[0]
+  }
 }
 
 template <class S>
@@ -306,14 +313,16 @@ void syreflect_upper(hipStream_t s, long N, double* C, long ldc) {
 
 template <class S>
 void gemm(hipStream_t s, int TA, int TB, long M, long N, long K, S alpha, const S* A, long lda,
-          const S* B, long ldb, S beta, S* C, long ldc, int mode, const int* skip) {
+          const S* B, long ldb, S beta, S* C, long ldc, int mode, const int* skip, int max_wg) {
   if (M <= 0 || N <= 0) return;
   if (K == 0 || alpha == S(0)) {
     scale<S>(s, M, N, beta, C, ldc, mode, skip);
     return;
   }
   const long tilesM = (M + BM - 1) / BM, tilesN = (N + BN - 1) / BN;
-  const dim3 g((unsigned)(tilesM * tilesN));
+  long nwg = tilesM * tilesN;
+  if (max_wg > 0 && nwg > max_wg) nwg = std::max(8L, (long)(max_wg / 8) * 8);
+  const dim3 g((unsigned)nwg);
   const bool at = is_trans(TA), bt = is_trans(TB);
   const bool va = aligned16<S>(A, lda), vb = aligned16<S>(B, ldb);
   if (!at && !bt)
@@ -334,8 +343,8 @@ void gemm(hipStream_t s, int TA, int TB, long M, long N, long K, S alpha, const 
 template void scale<double>(hipStream_t, long, long, double, double*, long, int, const int*);
 template void scale<float>(hipStream_t, long, long, float, float*, long, int, const int*);
 template void gemm<double>(hipStream_t, int, int, long, long, long, double, const double*, long,
-                           const double*, long, double, double*, long, int, const int*);
+                           const double*, long, double, double*, long, int, const int*, int);
 template void gemm<float>(hipStream_t, int, int, long, long, long, float, const float*, long,
-                          const float*, long, float, float*, long, int, const int*);
+                          const float*, long, float, float*, long, int, const int*, int);
 
 }  // namespace atlg

@@ -62,6 +62,13 @@ constexpr long kSpinLimit = 1L << 20;  // barrier polls (each one round trip) be
 #ifndef ATLG_LU_LOOKAHEAD
 #define ATLG_LU_LOOKAHEAD 0
 #endif
+#ifndef ATLG_LA_COOP
+#define ATLG_LA_COOP 0
+#endif
+// Grid cap of the look-ahead trailing update (workgroups; 0 = no cap).
+#ifndef ATLG_LA_GEMM_CAP
+#define ATLG_LA_GEMM_CAP 384
+#endif
 constexpr long kLaNB = 256;
 constexpr long kLaMin = 2048;
 
@@ -601,12 +608,15 @@ struct LU {
   std::vector<char> inv_done;
   int gemm_prec;     // kGemmNative or kGemmBf16 (float only)
   long win_lo, win_hi;  // leaves apply their interchanges to these columns only
+  bool coop = true;     // cooperative launch of the panel kernel
+  int gemm_cap = 0;     // grid cap of the trailing updates (0: none)
 };
 
 // Trailing update C -= A * B of a recursion node.
 inline void node_update(LU<double>& f, long m, long n, long k, const double* A, double* B,
                         double* C) {
-  gemm<double>(f.s, kNoTrans, kNoTrans, m, n, k, -1.0, A, f.lda, B, f.lda, 1.0, C, f.lda);
+  gemm<double>(f.s, kNoTrans, kNoTrans, m, n, k, -1.0, A, f.lda, B, f.lda, 1.0, C, f.lda, 0,
+               nullptr, f.gemm_cap);
 }
 inline void node_update(LU<float>& f, long m, long n, long k, const float* A, float* B,
                         float* C) {
@@ -614,7 +624,8 @@ inline void node_update(LU<float>& f, long m, long n, long k, const float* A, fl
     gemm_bf16(f.s, m, n, k, -1.0f, A, f.lda, B, f.lda, 1.0f, C, f.lda,
               f.gemm_prec == kGemmBf16x6);
   else
-    gemm<float>(f.s, kNoTrans, kNoTrans, m, n, k, -1.0f, A, f.lda, B, f.lda, 1.0f, C, f.lda);
+    gemm<float>(f.s, kNoTrans, kNoTrans, m, n, k, -1.0f, A, f.lda, B, f.lda, 1.0f, C, f.lda, 0,
+                nullptr, f.gemm_cap);
 }
 
 template <class S>
@@ -641,6 +652,16 @@ template <class S, int RPT>
 void launch_coop(LU<S>& f, int M, int w, int off, int G) {
   S* P = at(f, off, off);
   long lda = f.lda;
+  if (!f.coop) {
+    // Ordinary launch (look-ahead panel beside a running GEMM): the G <= 64
+    // workgroups fit the device many times over, so any of them still
+    // waiting for a slot gets one as the independent GEMM's workgroups
+    // retire; the barrier's bounded wait covers the rest.
+    hipLaunchKernelGGL((panel_coop<S, RPT>), dim3(G), dim3(CP_THREADS), 0, f.s, M, w, off, P, lda,
+                       f.ipiv, f.dinfo, f.cb, f.list);
+    ATLG_LAUNCHED();
+    return;
+  }
   void* args[] = {&M, &w, &off, &P, &lda, &f.ipiv, &f.dinfo, &f.cb, &f.list};
   ATLG_HIP(hipLaunchCooperativeKernel(reinterpret_cast<const void*>(&panel_coop<S, RPT>), dim3(G),
                                       dim3(CP_THREADS), args, 0, f.s));
@@ -799,6 +820,7 @@ void getrf_lookahead(LU<S>& f0) {
   hipEvent_t e_upd = aux_event(0), e_panel = aux_event(1);
   LU<S> f1;
   init_lu(f1, s1, f0.M, f0.N, f0.A, f0.lda, f0.ipiv, f0.dinfo, f0.gemm_prec);
+  f1.coop = ATLG_LA_COOP != 0;
   const long M = f0.M, N = f0.N, mn = std::min(M, N), lda = f0.lda;
   // block 0 on the caller's stream
   const long nb0 = std::min(kLaNB, mn);
@@ -820,10 +842,14 @@ void getrf_lookahead(LU<S>& f0) {
     f1.win_hi = k1 + nb1;
     getrf_rec(f1, k1, nb1);
     ATLG_HIP(hipEventRecord(e_panel, s1));
-    // the rest of step k's update, overlapping the panel
-    if (rest > 0)
+    // the rest of step k's update, overlapping the panel: a capped grid
+    // leaves compute units to the panel's workgroups
+    if (rest > 0) {
+      f0.gemm_cap = ATLG_LA_GEMM_CAP;
       node_update(f0, mrem, rest, kLaNB, at(f0, k1, k), at(f0, k, k1 + nb1),
                   at(f0, k1, k1 + nb1));
+      f0.gemm_cap = 0;
+    }
     ATLG_HIP(hipStreamWaitEvent(s0, e_panel, 0));
     if (rest > 0) {
       laswp<S>(s0, rest, at(f0, 0, k1 + nb1), lda, (int)k1, (int)(k1 + nb1), f0.ipiv, 1);

@@ -322,8 +322,8 @@ int atlg_sgetrs(int Trans, long N, long NRHS, const float* A, long lda, const in
 }
 
 int atlg_dsgesv(long N, long NRHS, double* A, long lda, int* ipiv, const double* B, long ldb,
-                double* X, long ldx, int* dinfo, int itermax, int lowprec, int* iter,
-                atlg_stream_t stream) {
+                double* X, long ldx, int* dinfo, int itermax, int lowprec, int refine, int* iter,
+                int* inner, atlg_stream_t stream) {
   if (N < 0) return -1;
   if (NRHS < 0) return -2;
   if (lda < std::max(1L, N)) return -4;
@@ -331,10 +331,11 @@ int atlg_dsgesv(long N, long NRHS, double* A, long lda, int* ipiv, const double*
   if (ldx < std::max(1L, N)) return -9;
   if (!dinfo) return -10;
   if (lowprec < 0 || lowprec > 2) return -12;
-  if (!iter) return -13;
+  if (refine != 0 && refine != 1) return -13;
+  if (!iter) return -14;
   return guarded([&] {
     dsgesv(resolve_stream(stream), N, NRHS, A, lda, ipiv, B, ldb, X, ldx, dinfo,
-           itermax > 0 ? itermax : 30, lowprec, iter);
+           itermax > 0 ? itermax : 30, lowprec, refine, iter, inner);
   });
 }
 

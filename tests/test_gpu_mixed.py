@@ -235,3 +235,38 @@ def test_dsgesv(G, lowprec, n, nrhs, kappa):
         assert it >= 0  # FP32-accurate factor: refinement converges for kappa * u32 < 1
     if kappa >= 1e10 and lowprec in (0, 2):
         assert it < 0
+
+
+@pytest.mark.parametrize("lowprec", [0, 1, 2])
+@pytest.mark.parametrize("n,nrhs,kappa", [(1024, 1, 1e2), (1536, 1, 1e6), (600, 2, 1e6),
+                                          (1024, 1, 1e10)])
+def test_dsgesv_gmres_ir(G, lowprec, n, nrhs, kappa):
+    """GMRES-IR (refine=1): the low-precision factor preconditions FP64
+    GMRES; every solve meets the FP64 backward-error bound (by refinement or
+    the FP64 fallback), converged ones with A and B untouched."""
+    import torch
+    rng = np.random.default_rng(int(7 * n + nrhs + np.log10(kappa)))
+    a = cond_matrix(n, kappa, rng)
+    b = rng.standard_normal((n, nrhs))
+    da, db = dev(a), dev(b)
+    dx = torch.zeros(n * nrhs, dtype=torch.float64, device="cuda")
+    ipiv = torch.zeros(n, dtype=torch.int32, device="cuda")
+    info = torch.zeros(1, dtype=torch.int32, device="cuda")
+    it, inner = G.dsgesv(n, nrhs, da, n, ipiv, db, n, dx, n, info, 30, lowprec, refine=1,
+                         inner=True)
+    torch.cuda.synchronize()
+    assert int(info.item()) == 0
+    x = host(dx, n, nrhs, n)
+    be = backward_error(a, x, b)
+    print("gmres-ir lowprec=%d n=%d nrhs=%d kappa=%g: ITER=%d inner=%d backward error %.3g"
+          % (lowprec, n, nrhs, kappa, it, inner, be))
+    assert be <= 1e-15 * n
+    assert np.array_equal(host(db, n, nrhs, n), b)
+    if it >= 0:
+        assert np.array_equal(host(da, n, n, n), a)
+    # GMRES-IR converges while kappa * u_factor stays well below 1/u_factor:
+    # FP32-accurate factors (FP32, split BF16) to kappa = 1e6; the BF16
+    # factor (u = 2^-8, beyond the reach of classical refinement already at
+    # kappa = 1e2 for larger n) at kappa = 1e2
+    if kappa <= (1e6 if lowprec in (0, 2) else 1e2):
+        assert it >= 0
