@@ -496,7 +496,8 @@ def main():
     del a, b, c
     torch.cuda.empty_cache()
     if not args.no_sweep:
-        out["sweep"] = {k: (round(v, 4) if isinstance(v, float) else v)
+        # five significant digits: backward errors sit near 1e-15
+        out["sweep"] = {k: (float("%.5g" % v) if isinstance(v, float) else v)
                         for k, v in sweep(torch, G, "", args.ir_n).items()}
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(n, args.cpu_cols)

@@ -38,19 +38,24 @@ __device__ inline S opa(const S* A, long lda, bool trans, long i, long j) {
 // columns [b*NB, b*NB + nbb).  An upper-triangular op(A) block U is handled
 // as the lower-triangular J U J (J = order reversal), so one in-place lower
 // inversion serves all cases.  Output: Dinv + b*NB*NB, column-major, ld NB,
-// zeros outside the triangle; Unit diagonals are never read.
-template <class R>
+// zeros outside the triangle; Unit diagonals are never read.  NT is the
+// order held in LDS: NB for the general case; 32 or 64 when na <= NT (one
+// block), which then writes only the leading NT x NT part of the block (the
+// solves read the leading na x na part) and needs a fraction of the LDS, so
+// the launch fits beside GEMM workgroups on a busy device.
+template <class R, int NT>
 __global__ void __launch_bounds__(TRI_THREADS)
 trtri_diag_kernel(int na, const R* __restrict__ A, long lda, int trans, int lower, int unit,
                   R* __restrict__ Dinv, const int* __restrict__ skip) {
   if (skip && *skip) return;
-  __shared__ R S[kTriN * kTriLD];
+  constexpr int LD = NT + 1;
+  __shared__ R S[NT * LD];
   const int b = blockIdx.x, t = threadIdx.x;
   const long o = (long)b * NB;
-  const int n = (int)min((long)NB, na - o);
+  const int n = (int)min((long)NT, na - o);
   const bool tr = trans != 0;
-  for (int idx = t; idx < NB * NB; idx += TRI_THREADS) {
-    const int i = idx % NB, j = idx / NB;
+  for (int idx = t; idx < NT * NT; idx += TRI_THREADS) {
+    const int i = idx % NT, j = idx / NT;
     R v = R(0);
     if (i >= j) {
       if (i < n && j < n) {
@@ -61,17 +66,17 @@ trtri_diag_kernel(int na, const R* __restrict__ A, long lda, int trans, int lowe
         v = R(1);
       }
     }
-    S[i * kTriLD + j] = v;
+    S[i * LD + j] = v;
   }
   __syncthreads();
-  lower_inverse_inplace(S, n);
+  lower_inverse_inplace<R, NT>(S, n);
   R* D = Dinv + (size_t)b * NB * NB;
-  for (int idx = t; idx < NB * NB; idx += TRI_THREADS) {
-    const int i = idx % NB, j = idx / NB;
+  for (int idx = t; idx < NT * NT; idx += TRI_THREADS) {
+    const int i = idx % NT, j = idx / NT;
     R v = R(0);
     if (i < n && j < n) {
-      if (lower) v = (i >= j) ? S[i * kTriLD + j] : R(0);
-      else v = (i <= j) ? S[(n - 1 - i) * kTriLD + (n - 1 - j)] : R(0);
+      if (lower) v = (i >= j) ? S[i * LD + j] : R(0);
+      else v = (i <= j) ? S[(n - 1 - i) * LD + (n - 1 - j)] : R(0);
     }
     D[i + (size_t)j * NB] = v;
   }
@@ -348,9 +353,16 @@ void tri_inverse_blocks(hipStream_t s, long na, const S* A, long lda, int ta, in
   if (na <= 0) return;
   const bool tr = is_trans(ta);
   const long nblk = (na + NB - 1) / NB;
-  hipLaunchKernelGGL(trtri_diag_kernel<S>, dim3((unsigned)nblk), dim3(TRI_THREADS), 0, s, (int)na,
-                     A, lda, tr ? 1 : 0, (uplo == kLower) != tr ? 1 : 0, diag == kUnit ? 1 : 0,
-                     Dinv, skip);
+  const int trn = tr ? 1 : 0, low = (uplo == kLower) != tr ? 1 : 0, un = diag == kUnit ? 1 : 0;
+  if (na <= 32)
+    hipLaunchKernelGGL((trtri_diag_kernel<S, 32>), dim3(1), dim3(TRI_THREADS), 0, s, (int)na, A,
+                       lda, trn, low, un, Dinv, skip);
+  else if (na <= 64)
+    hipLaunchKernelGGL((trtri_diag_kernel<S, 64>), dim3(1), dim3(TRI_THREADS), 0, s, (int)na, A,
+                       lda, trn, low, un, Dinv, skip);
+  else
+    hipLaunchKernelGGL((trtri_diag_kernel<S, NB>), dim3((unsigned)nblk), dim3(TRI_THREADS), 0, s,
+                       (int)na, A, lda, trn, low, un, Dinv, skip);
   ATLG_LAUNCHED();
 }
 
