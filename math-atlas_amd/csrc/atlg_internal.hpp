@@ -77,7 +77,8 @@ enum Slot : int {
   kSlotMixed = 13,      // FP32 factor, residual partials and norms of dsgesv
   kSlotBf16 = 14,       // BF16 operand copies of the low-precision update
   kSlotTrsv = 15,       // ready flags of the few-RHS triangular solve
-  kNumSlots = 16
+  kSlotLaLists = 16,    // per-leaf interchange lists of the look-ahead LU
+  kNumSlots = 17
 };
 void* scratch(hipStream_t s, Slot slot, size_t bytes);
 

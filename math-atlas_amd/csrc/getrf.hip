@@ -19,11 +19,12 @@
 //  * A leaf panel is factored by one cooperative launch (panel_coop): the
 //    panel rows live in the registers of G <= 64 workgroups; per column each
 //    workgroup finds its candidate pivot (wavefront argmax over cross-lane
-//    shuffles, then LDS), publishes it with the candidate row, and after one
-//    grid barrier every workgroup picks the same global pivot, swaps, scales
-//    and applies the rank-1 update to its rows.  The barrier's wait is
-//    bounded; on overrun every workgroup leaves and *dinfo = INT_MIN reports
-//    the failure.
+//    shuffles, then LDS) and publishes it with the candidate row into the
+//    column's write-once mailbox; every workgroup polls the mailbox slots
+//    until all are written (one round trip, no barrier), picks the same
+//    global pivot, swaps, scales and applies the rank-1 update to its rows.
+//    The waits are bounded; on overrun every workgroup leaves and
+//    *dinfo = INT_MIN reports the failure.
 //
 // Pivot choice follows cblas_idamax (AtlasBase/Antoine/atlrefblas.base:753+):
 // first index of the largest |a|, NaN never chosen; the column is scaled by
@@ -51,7 +52,8 @@ constexpr int CP_MAXG = 64;       // workgroups of the cooperative panel
 constexpr int STEP_THREADS = 256;
 constexpr int KC = 256;           // interchanges composed per laswp chunk
 constexpr int CW = 16;            // columns per laswp workgroup
-constexpr long kSpinLimit = 1L << 20;  // barrier polls (each one round trip) before giving up
+constexpr int kLeafListInts = 1 + 4 * PW;  // one leaf's composed list: count + pairs
+constexpr long kSpinLimit = 1L << 20;  // mailbox polls (each one round trip) before giving up
 // Fewest panel rows per thread of the cooperative panel (1, 2 or 4): more
 // rows per thread means fewer workgroups exchanging candidates per column.
 #ifndef ATLG_PANEL_RPT_MIN
@@ -60,10 +62,15 @@ constexpr long kSpinLimit = 1L << 20;  // barrier polls (each one round trip) be
 // Look-ahead driver (1) or ATLAS's recursion over the whole matrix (0), and
 // its block-column width and the smallest min(M, N) that uses it.
 #ifndef ATLG_LU_LOOKAHEAD
-#define ATLG_LU_LOOKAHEAD 0
+#define ATLG_LU_LOOKAHEAD 1
 #endif
 #ifndef ATLG_LA_COOP
 #define ATLG_LA_COOP 0
+#endif
+// Wave priority of the panel kernel (0-3): above the GEMM waves it shares
+// SIMDs with while the look-ahead overlaps them.
+#ifndef ATLG_PANEL_PRIO
+#define ATLG_PANEL_PRIO 3
 #endif
 // Grid cap of the look-ahead trailing update (workgroups; 0 = no cap).
 #ifndef ATLG_LA_GEMM_CAP
@@ -266,77 +273,151 @@ laswp_apply(long N, S* __restrict__ A, long lda, const int* __restrict__ lst) {
   }
 }
 
-// ---------------------------------------------------------------- panels
-
-// Global scratch of one cooperative panel launch.
+// Applies the composed lists of consecutive leaves (lists[l], l < nl, each
+// kLeafListInts ints, in order) to the columns [c0, c0 + n0) and
+// [c1, c1 + n1): per list, gather the source rows into LDS, then scatter.
+// The same workgroup reads what the previous list wrote, so the barrier
+// between lists is enough.
 template <class S>
-struct CoopBuf {
-  unsigned bar;  // barrier arrivals (reset to 0 before each launch)
-  int err;       // 1 when a barrier wait ran past its bound
-  int pad[2];
-  Cand cand[2][CP_MAXG];           // per column parity: workgroup candidates
-  S crow[2][CP_MAXG][PW];          // ... and their candidate rows
-  S jrow[2][PW];                   // ... and row j
-};
-
-template <class S>
-__global__ void coop_reset(CoopBuf<S>* cb, int* list) {
-  cb->bar = 0u;
-  cb->err = 0;
-  list[0] = 0;
+__global__ void __launch_bounds__(256)
+laswp_lists(S* __restrict__ A, long lda, long c0, long n0, long c1, long n1,
+            const int* __restrict__ lists, int nl) {
+  __shared__ S buf[2 * PW * CW];
+  const long nb0 = (n0 + CW - 1) / CW, b = blockIdx.x;
+  const long cb = b < nb0 ? c0 + b * CW : c1 + (b - nb0) * CW;
+  const long ce = b < nb0 ? c0 + n0 : c1 + n1;
+  const int nc = (int)min((long)CW, ce - cb);
+  for (int l = 0; l < nl; l++) {
+    const int* lst = lists + (size_t)l * kLeafListInts;
+    const int T = lst[0];
+    for (int e = threadIdx.x; e < T * CW; e += blockDim.x) {
+      const int t = e / CW, c = e % CW;
+      if (c < nc) buf[e] = A[(size_t)lst[2 + 2 * t] + (size_t)(cb + c) * lda];
+    }
+    __syncthreads();
+    for (int e = threadIdx.x; e < T * CW; e += blockDim.x) {
+      const int t = e / CW, c = e % CW;
+      if (c < nc) A[(size_t)lst[1 + 2 * t] + (size_t)(cb + c) * lda] = buf[e];
+    }
+    __syncthreads();
+  }
 }
 
-// Grid barrier of a cooperative launch: returns once `target` arrivals are
-// counted.  All data exchanged between workgroups goes through agent-scope
-// (coherence-point) atomic stores and loads, so the barrier itself needs no
-// cache write-back or invalidation: writers wait for their stores to
-// complete (workgroup-scope release), thread 0 counts the arrival and polls
-// the counter with relaxed loads.  The verdict is decided by thread 0 and
-// broadcast through LDS so that every thread of the workgroup takes the same
-// branch.  Returns false when the wait ran past its bound (in any
-// workgroup).
+// ---------------------------------------------------------------- panels
+
+// Words of the column mailboxes, handled as raw bits: a slot holding the
+// sentinel has not been written yet.  The sentinel is a signalling-NaN
+// pattern; a value with exactly that pattern is published as the quiet NaN
+// (only the payload changes), so no written slot ever reads as empty.
 template <class S>
-__device__ bool grid_sync(CoopBuf<S>* cb, unsigned target, int* verdict) {
-  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
-  __syncthreads();
-  if (threadIdx.x == 0) {
-    __hip_atomic_fetch_add(&cb->bar, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    // Each poll is one round trip to the coherence point, which paces the
-    // loop; the error word (another workgroup gave up) is read only every
-    // 256 polls so that the common case costs a single round trip.
-    bool ok = true;
-    long spins = 0;
-    while (__hip_atomic_load(&cb->bar, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < target) {
-      if ((++spins & 255) == 0 && (ld_agent(&cb->err) || spins > kSpinLimit)) {
-        st_agent(&cb->err, 1);
-        ok = false;
-        break;
-      }
-    }
-    *verdict = ok;
+struct Word;
+template <>
+struct Word<double> {
+  typedef unsigned long long U;
+  static constexpr U kSent = 0x7FF4A5A5A5A5A5A5ull;
+  static constexpr U kQNaN = 0x7FF8000000000000ull;
+  __device__ static U bits(double x) { return static_cast<U>(__double_as_longlong(x)); }
+  __device__ static double val(U u) { return __longlong_as_double(static_cast<long long>(u)); }
+};
+template <>
+struct Word<float> {
+  typedef unsigned U;
+  static constexpr U kSent = 0x7FA5A5A5u;
+  static constexpr U kQNaN = 0x7FC00000u;
+  __device__ static U bits(float x) { return __float_as_uint(x); }
+  __device__ static float val(U u) { return __uint_as_float(u); }
+};
+constexpr unsigned long long kKeySent = Word<double>::kSent;  // keys are never NaN
+constexpr unsigned kIdxSent = 0xFFFFFFFFu;                    // rows are never that
+
+template <class U>
+__device__ inline U ld_bits(const U* p) {
+  return __hip_atomic_load(const_cast<U*>(p), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+template <class U>
+__device__ inline void st_bits(U* p, U v) {
+  __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+template <class S>
+__device__ inline typename Word<S>::U pub_bits(S x) {
+  const typename Word<S>::U u = Word<S>::bits(x);
+  return u == Word<S>::kSent ? Word<S>::kQNaN : u;
+}
+
+// One leaf's exchange area: per column j and workgroup g the candidate
+// (key, row) and its row, plus row j.  Every slot is written at most once
+// per leaf, so a reader simply polls the slots it needs until none holds
+// the sentinel.  Two areas alternate between consecutive leaves: a leaf
+// refills the other one with sentinels for the next leaf (the previous
+// leaf, which used it, has finished: launches on a stream are ordered).
+template <class S>
+struct Mailbox {
+  typedef typename Word<S>::U U;
+  unsigned long long key[PW][CP_MAXG];
+  unsigned idx[PW][CP_MAXG];
+  U crow[PW][CP_MAXG][PW];
+  U jrow[PW][PW];
+  int err;  // 1 when a wait ran past its bound (in any workgroup)
+};
+
+// Global scratch of the cooperative panel: the two alternating mailboxes.
+template <class S>
+struct CoopBuf {
+  Mailbox<S> mb[2];
+};
+
+// Fills mailbox `m` with sentinels for the workgroups g < gclear (the
+// previous user's grid) and clears its error word; the calling grid's
+// threads share the work (thread gt of nt).
+template <class S>
+__device__ void mb_clear(Mailbox<S>* m, int gclear, int gt, int nt) {
+  for (int e = gt; e < PW * gclear; e += nt) {
+    const int j = e / gclear, g = e - j * gclear;
+    m->key[j][g] = kKeySent;
+    m->idx[j][g] = kIdxSent;
   }
-  __syncthreads();
-  return *verdict != 0;
+  const int per = gclear * PW;
+  for (int e = gt; e < PW * per; e += nt) {
+    const int j = e / per, r = e - j * per;
+    m->crow[j][r / PW][r % PW] = Word<S>::kSent;
+  }
+  for (int e = gt; e < PW * PW; e += nt) m->jrow[e / PW][e % PW] = Word<S>::kSent;
+  if (gt == 0) m->err = 0;
+}
+
+template <class S>
+__global__ void mb_init(CoopBuf<S>* cb) {
+  const int gt = blockIdx.x * blockDim.x + threadIdx.x, nt = gridDim.x * blockDim.x;
+  mb_clear(&cb->mb[0], CP_MAXG, gt, nt);
+  mb_clear(&cb->mb[1], CP_MAXG, gt, nt);
 }
 
 // Factors the M x w (w <= PW) panel P (whose row 0 is absolute row `off` of
 // the matrix, on the diagonal).  Row i of the panel is held by global thread
 // i mod (G*256), RPT rows per thread.  Writes ipiv[off + j] (absolute),
-// *dinfo (first zero pivot), and the composed interchange list.
+// *dinfo (first zero pivot), and the composed interchange list.  With
+// G > 1 the workgroups exchange through mailbox `par` (and refill mailbox
+// par^1, last used by a grid of gclear workgroups); G == 1 stays in LDS.
 template <class S, int RPT>
 __global__ void __launch_bounds__(CP_THREADS)
 panel_coop(int M, int w, int off, S* __restrict__ P, long lda, int* __restrict__ ipiv,
-           int* __restrict__ dinfo, CoopBuf<S>* __restrict__ cb, int* __restrict__ list) {
+           int* __restrict__ dinfo, CoopBuf<S>* __restrict__ cb, int par, int gclear,
+           int* __restrict__ list) {
+  typedef typename Word<S>::U U;
+  if (ATLG_PANEL_PRIO > 0) __builtin_amdgcn_s_setprio(ATLG_PANEL_PRIO);
   __shared__ S prow[PW], jrow[PW];
   __shared__ S crows[CP_MAXG * PW];
   __shared__ S stage_c[PW], stage_j[PW];  // this workgroup's candidate row / row j
-  __shared__ int has_c, has_j;
+  __shared__ int has_c, has_j, sbad;
   __shared__ double sk[CP_THREADS / 64];
   __shared__ int si[CP_THREADS / 64];
-  __shared__ int winner, verdict;
+  __shared__ int winner;
   __shared__ int pa[PW], pb[PW], cval[2 * PW], ccnt;
   const int t = threadIdx.x, G = gridDim.x, GT = G * CP_THREADS;
   const int g = blockIdx.x * CP_THREADS + t;
+  Mailbox<S>* mb = &cb->mb[par];
+  if (G > 1) mb_clear(&cb->mb[par ^ 1], gclear, g, GT);
+  if (t == 0) sbad = 0;
   S v[RPT][PW];
 #pragma unroll
   for (int q = 0; q < RPT; q++) {
@@ -347,7 +428,6 @@ panel_coop(int M, int w, int off, S* __restrict__ P, long lda, int* __restrict__
   const int mn = min(M, w);
   bool ok = true;
   for (int j = 0; j < mn; j++) {
-    const int par = j & 1;
     double key = -2.0;
     int idx = INT_MAX;
 #pragma unroll
@@ -379,72 +459,104 @@ panel_coop(int M, int w, int off, S* __restrict__ P, long lda, int* __restrict__
       if (t == 0) winner = idx;
       __syncthreads();
     } else {
-    // Publish: the owners copy the candidate row and row j to LDS, then one
-    // lane per element stores them (an agent-scope atomic store waits for
-    // the stores before it, so one lane storing a whole row would pay a
-    // round trip per element).
-    if (t == 0) {
-      has_c = 0;
-      has_j = 0;
-    }
-    __syncthreads();
-#pragma unroll
-    for (int q = 0; q < RPT; q++) {
-      const int i = g + q * GT;
-      if (i == idx) {
-#pragma unroll
-        for (int c = 0; c < PW; c++) stage_c[c] = v[q][c];
-        has_c = 1;
+      // Publish: the owners copy the candidate row and row j to LDS, then one
+      // lane per element stores them (an agent-scope atomic store waits for
+      // the stores before it, so one lane storing a whole row would pay a
+      // round trip per element).  A workgroup without rows >= j publishes
+      // key -2 and a zero row (never chosen: rows j.. always exist).
+      if (t == 0) {
+        has_c = 0;
+        has_j = 0;
       }
-      if (i == j) {
+      __syncthreads();
 #pragma unroll
-        for (int c = 0; c < PW; c++) stage_j[c] = v[q][c];
-        has_j = 1;
+      for (int q = 0; q < RPT; q++) {
+        const int i = g + q * GT;
+        if (i == idx) {
+#pragma unroll
+          for (int c = 0; c < PW; c++) stage_c[c] = v[q][c];
+          has_c = 1;
+        }
+        if (i == j) {
+#pragma unroll
+          for (int c = 0; c < PW; c++) stage_j[c] = v[q][c];
+          has_j = 1;
+        }
       }
-    }
-    __syncthreads();
-    if (t < 64) {
-      const int c = t & (PW - 1);
+      __syncthreads();
       if (t < PW) {
-        if (has_c && c < w) st_agent(&cb->crow[par][blockIdx.x][c], stage_c[c]);
-      } else if (has_j && c < w) {
-        st_agent(&cb->jrow[par][c], stage_j[c]);
+        if (t < w) st_bits(&mb->crow[j][blockIdx.x][t], pub_bits(has_c ? stage_c[t] : S(0)));
+      } else if (t < 2 * PW) {
+        if (has_j && t - PW < w) st_bits(&mb->jrow[j][t - PW], pub_bits(stage_j[t - PW]));
+      } else if (t == 2 * PW) {
+        st_bits(&mb->key[j][blockIdx.x], Word<double>::bits(key));
+      } else if (t == 2 * PW + 1) {
+        st_bits(&mb->idx[j][blockIdx.x], static_cast<unsigned>(idx));
       }
-    }
-    if (t == 64) {
-      st_agent(&cb->cand[par][blockIdx.x].key, key);
-      st_agent(&cb->cand[par][blockIdx.x].idx, idx);
-    }
-    if (!grid_sync(cb, (unsigned)G * (j + 1), &verdict)) {
-      ok = false;
-      break;
-    }
-    // one round trip: every workgroup reads all G candidates, all G
-    // candidate rows and row j, then picks the winner locally
-    {
-      constexpr int NU = CP_MAXG * PW / CP_THREADS;  // loads per thread, all in flight
-      S tmp[NU];
+      // Gather: all G candidate rows (8 slots per thread, all loads in
+      // flight), the G candidates (threads < G) and row j (threads 128..),
+      // each slot polled until written; one round trip once the last
+      // workgroup has published.
+      constexpr int NU = CP_MAXG * PW / CP_THREADS;
+      U cw[NU];
+#pragma unroll
+      for (int u = 0; u < NU; u++) cw[u] = Word<S>::kSent;
+      unsigned long long kw = kKeySent;
+      unsigned iw = kIdxSent;
+      U jw = Word<S>::kSent;
+      const bool want_k = t < G, want_j = t >= 4 * PW && t - 4 * PW < w;
+      long spins = 0;
+      for (;;) {
+        // issue every pending load before looking at any result, so that
+        // they are all in flight together (one round trip per poll)
+        bool pend[NU];
+#pragma unroll
+        for (int u = 0; u < NU; u++) {
+          const int e = t + u * CP_THREADS, c = e % PW;
+          pend[u] = e < G * PW && c < w && cw[u] == Word<S>::kSent;
+        }
+        const bool pk = want_k && kw == kKeySent, pi = want_k && iw == kIdxSent;
+        const bool pj = want_j && jw == Word<S>::kSent;
+#pragma unroll
+        for (int u = 0; u < NU; u++) {
+          const int e = t + u * CP_THREADS, bq = e / PW, c = e - bq * PW;
+          if (pend[u]) cw[u] = ld_bits(&mb->crow[j][bq][c]);
+        }
+        if (pk) kw = ld_bits(&mb->key[j][t]);
+        if (pi) iw = ld_bits(&mb->idx[j][t]);
+        if (pj) jw = ld_bits(&mb->jrow[j][t - 4 * PW]);
+        bool waiting = (want_k && (kw == kKeySent || iw == kIdxSent)) ||
+                       (want_j && jw == Word<S>::kSent);
+#pragma unroll
+        for (int u = 0; u < NU; u++) {
+          const int e = t + u * CP_THREADS, c = e % PW;
+          waiting |= e < G * PW && c < w && cw[u] == Word<S>::kSent;
+        }
+        if (!waiting) break;
+        // the error word (another workgroup gave up) is read every 256 polls
+        if ((++spins & 255) == 0 && (ld_agent(&mb->err) || spins > kSpinLimit)) {
+          st_agent(&mb->err, 1);
+          sbad = 1;
+          break;
+        }
+      }
 #pragma unroll
       for (int u = 0; u < NU; u++) {
-        const int e = t + u * CP_THREADS, bq = e / PW, c = e - bq * PW;
-        tmp[u] = (e < G * PW && c < w) ? ld_agent(&cb->crow[par][bq][c]) : S(0);
+        const int e = t + u * CP_THREADS, c = e % PW;
+        if (e < G * PW) crows[e] = c < w ? Word<S>::val(cw[u]) : S(0);
       }
-#pragma unroll
-      for (int u = 0; u < NU; u++)
-        if (t + u * CP_THREADS < G * PW) crows[t + u * CP_THREADS] = tmp[u];
-    }
-    if (t < PW) jrow[t] = (t < w) ? ld_agent(&cb->jrow[par][t]) : S(0);
-    if (t < 64) {
-      double kk = -3.0;
-      int ii = INT_MAX;
-      if (t < G) {
-        kk = ld_agent(&cb->cand[par][t].key);
-        ii = ld_agent(&cb->cand[par][t].idx);
+      if (t >= 4 * PW && t < 5 * PW) jrow[t - 4 * PW] = want_j ? Word<S>::val(jw) : S(0);
+      if (t < 64) {
+        double kk = want_k ? Word<double>::val(kw) : -3.0;
+        int ii = want_k ? static_cast<int>(iw) : INT_MAX;
+        wave_argmax(kk, ii);
+        if (t == 0) winner = ii;
       }
-      wave_argmax(kk, ii);
-      if (t == 0) winner = ii;
-    }
-    __syncthreads();
+      __syncthreads();
+      if (sbad) {
+        ok = false;
+        break;
+      }
     }  // G > 1
     // rows j..M-1 always hold a candidate, so p is in [j, M); the clamp only
     // keeps a corrupted exchange from indexing outside the panel
@@ -500,6 +612,7 @@ panel_coop(int M, int w, int off, S* __restrict__ P, long lda, int* __restrict__
   }
   if (!ok) {
     if (t == 0) *dinfo = INT_MIN;
+    if (blockIdx.x == 0 && t == 0) list[0] = 0;  // apply no interchanges
     return;
   }
   if (blockIdx.x == 0) {
@@ -610,6 +723,9 @@ struct LU {
   long win_lo, win_hi;  // leaves apply their interchanges to these columns only
   bool coop = true;     // cooperative launch of the panel kernel
   int gemm_cap = 0;     // grid cap of the trailing updates (0: none)
+  int* leaf_lists = nullptr;  // look-ahead: leaf at column c keeps its list at c/PW
+  int mb_par = 0;             // mailbox the next multi-workgroup panel uses
+  int mb_g[2] = {0, 0};       // grid of each mailbox's last user (0: clean)
 };
 
 // Trailing update C -= A * B of a recursion node.
@@ -649,20 +765,29 @@ void apply_outside(LU<S>& f, long c0, long w, const int* lst) {
 }
 
 template <class S, int RPT>
-void launch_coop(LU<S>& f, int M, int w, int off, int G) {
+void launch_coop(LU<S>& f, int M, int w, int off, int G, int* lst) {
   S* P = at(f, off, off);
   long lda = f.lda;
+  // a multi-workgroup panel takes the next mailbox and refills the other
+  int par = 0, gclear = 0;
+  if (G > 1) {
+    par = f.mb_par;
+    gclear = f.mb_g[par ^ 1];
+    f.mb_g[par ^ 1] = 0;
+    f.mb_g[par] = G;
+    f.mb_par = par ^ 1;
+  }
   if (!f.coop) {
     // Ordinary launch (look-ahead panel beside a running GEMM): the G <= 64
     // workgroups fit the device many times over, so any of them still
     // waiting for a slot gets one as the independent GEMM's workgroups
-    // retire; the barrier's bounded wait covers the rest.
+    // retire; the mailbox waits are bounded anyway.
     hipLaunchKernelGGL((panel_coop<S, RPT>), dim3(G), dim3(CP_THREADS), 0, f.s, M, w, off, P, lda,
-                       f.ipiv, f.dinfo, f.cb, f.list);
+                       f.ipiv, f.dinfo, f.cb, par, gclear, lst);
     ATLG_LAUNCHED();
     return;
   }
-  void* args[] = {&M, &w, &off, &P, &lda, &f.ipiv, &f.dinfo, &f.cb, &f.list};
+  void* args[] = {&M, &w, &off, &P, &lda, &f.ipiv, &f.dinfo, &f.cb, &par, &gclear, &lst};
   ATLG_HIP(hipLaunchCooperativeKernel(reinterpret_cast<const void*>(&panel_coop<S, RPT>), dim3(G),
                                       dim3(CP_THREADS), args, 0, f.s));
 }
@@ -671,16 +796,15 @@ void launch_coop(LU<S>& f, int M, int w, int off, int G) {
 // interchanges to every other column.
 template <class S>
 void leaf(LU<S>& f, long off, long m, int w) {
+  int* lst = f.leaf_lists ? f.leaf_lists + (off / PW) * kLeafListInts : f.list;
   const long cap = (long)CP_MAXG * CP_THREADS;
   if (m <= 4 * cap) {
     const int rpt = std::max(ATLG_PANEL_RPT_MIN, m <= cap ? 1 : (m <= 2 * cap ? 2 : 4));
     const int G = (int)((m + (long)CP_THREADS * rpt - 1) / ((long)CP_THREADS * rpt));
-    hipLaunchKernelGGL(coop_reset<S>, dim3(1), dim3(1), 0, f.s, f.cb, f.list);
-    ATLG_LAUNCHED();
-    if (rpt == 1) launch_coop<S, 1>(f, (int)m, w, (int)off, G);
-    else if (rpt == 2) launch_coop<S, 2>(f, (int)m, w, (int)off, G);
-    else launch_coop<S, 4>(f, (int)m, w, (int)off, G);
-    apply_outside(f, off, w, f.list);
+    if (rpt == 1) launch_coop<S, 1>(f, (int)m, w, (int)off, G, lst);
+    else if (rpt == 2) launch_coop<S, 2>(f, (int)m, w, (int)off, G, lst);
+    else launch_coop<S, 4>(f, (int)m, w, (int)off, G, lst);
+    apply_outside(f, off, w, lst);
     return;
   }
   const int mn = (int)std::min<long>(m, w);
@@ -692,14 +816,16 @@ void leaf(LU<S>& f, long off, long m, int w) {
     ATLG_LAUNCHED();
   }
   hipLaunchKernelGGL(laswp_compose, dim3(1), dim3(KC), 0, f.s, f.ipiv, (int)off, (int)off + mn,
-                     1, f.list);
+                     1, lst);
   ATLG_LAUNCHED();
-  apply_outside(f, off, w, f.list);
+  apply_outside(f, off, w, lst);
 }
 
-// U12 <- L11^-1 A12 for the node at `off` with nl pivot columns.
+// U12 <- L11^-1 A12 for the node at `off` with nl pivot columns, on the nc
+// columns starting at column c0 (c0 >= off + nl).
 template <class S>
-void node_trsm(LU<S>& f, long off, long nl, long nr) {
+void node_trsm_cols(LU<S>& f, long off, long nl, long c0, long nc) {
+  if (nc <= 0) return;
   if (nl >= kTrsmNB && off % kTrsmNB == 0 && nl % kTrsmNB == 0) {
     const long b0 = off / kTrsmNB, b1 = (off + nl) / kTrsmNB;
     long first = b0;
@@ -710,12 +836,17 @@ void node_trsm(LU<S>& f, long off, long nl, long nr) {
                          f.Winv + (size_t)first * kTrsmNB * kTrsmNB);
       for (long b = first; b < b1; b++) f.inv_done[b] = 1;
     }
-    trsm<S>(f.s, kLeft, kLower, kNoTrans, kUnit, nl, nr, S(1), at(f, off, off), f.lda,
-            at(f, off, off + nl), f.lda, nullptr, f.Winv + (size_t)b0 * kTrsmNB * kTrsmNB, 0);
+    trsm<S>(f.s, kLeft, kLower, kNoTrans, kUnit, nl, nc, S(1), at(f, off, off), f.lda,
+            at(f, off, c0), f.lda, nullptr, f.Winv + (size_t)b0 * kTrsmNB * kTrsmNB, 0);
   } else {
-    trsm<S>(f.s, kLeft, kLower, kNoTrans, kUnit, nl, nr, S(1), at(f, off, off), f.lda,
-            at(f, off, off + nl), f.lda);
+    trsm<S>(f.s, kLeft, kLower, kNoTrans, kUnit, nl, nc, S(1), at(f, off, off), f.lda,
+            at(f, off, c0), f.lda);
   }
+}
+
+template <class S>
+void node_trsm(LU<S>& f, long off, long nl, long nr) {
+  node_trsm_cols(f, off, nl, off + nl, nr);
 }
 
 // Factors rows [off, M) x columns [off, off + n).
@@ -800,65 +931,85 @@ void init_lu(LU<S>& f, hipStream_t s, long M, long N, S* A, long lda, int* ipiv,
   f.rec = reinterpret_cast<PivRec<S>*>(base + b_cb + b_list + 256);
   f.cand = reinterpret_cast<Cand*>(base + b_cb + b_list + 256 + b_rec);
   set_int(s, reinterpret_cast<int*>(f.ticket), 0);
+  hipLaunchKernelGGL(mb_init<S>, dim3(128), dim3(256), 0, s, f.cb);
+  ATLG_LAUNCHED();
+  f.mb_par = 0;
+  f.mb_g[0] = f.mb_g[1] = 0;
   const long nblk = (std::min(M, N) + kTrsmNB - 1) / kTrsmNB;
   f.Winv = static_cast<S*>(
       scratch(s, kSlotFactInv, sizeof(S) * std::max<long>(nblk, 1) * kTrsmNB * kTrsmNB));
   f.inv_done.assign(nblk + 1, 0);
 }
 
-// Right-looking LU by kLaNB-wide block columns with one step of look-ahead:
-// while the caller's stream applies step k's trailing update to the columns
-// right of block k+1, the auxiliary stream factors block k+1 (updated
-// first).  Each block is factored by ATLAS's recursion (getrf_rec) with its
-// leaves' interchanges confined to the block; the block's interchanges reach
-// the columns to its right after the join and the columns to its left at the
-// end.  Partial pivoting picks from the same updated columns as the plain
-// recursion, so the pivots agree wherever they are well separated.
+// Applies the interchanges of the leaves covering columns [k0, k0 + kb) to
+// the columns [c0, c0 + n0) and [c1, c1 + n1), in pivot order.
+template <class S>
+void apply_block_lists(LU<S>& f, long k0, long kb, long c0, long n0, long c1, long n1) {
+  n0 = std::max(0L, n0);
+  n1 = std::max(0L, n1);
+  const long nblk = (n0 + CW - 1) / CW + (n1 + CW - 1) / CW;
+  if (nblk == 0) return;
+  hipLaunchKernelGGL(laswp_lists<S>, dim3((unsigned)nblk), dim3(256), 0, f.s, f.A, f.lda, c0, n0,
+                     c1, n1, f.leaf_lists + (k0 / PW) * kLeafListInts, (int)((kb + PW - 1) / PW));
+  ATLG_LAUNCHED();
+}
+
+// Right-looking LU by kLaNB-wide block columns with one step of look-ahead.
+// Block k is factored by ATLAS's recursion (getrf_rec) with its leaves'
+// interchanges confined to the block; every leaf keeps its composed list.
+// Step k then, on the caller's stream:
+//   1. brings block k+1 up to date: block k's interchanges, U = L_kk^-1 A,
+//      and the GEMM update (the critical path);
+//   2. hands block k+1 to the auxiliary stream, which factors it;
+//   3. meanwhile applies block k's interchanges to the columns left of block
+//      k and right of block k+1, and their TRSM and GEMM update (the GEMM on
+//      a capped grid, which leaves compute units to the panel);
+//   4. waits for the panel before step k+1.
+// The columns each stream touches are disjoint while both run.  Partial
+// pivoting picks from the same updated columns as the plain recursion, so
+// the pivots agree wherever they are well separated.
 template <class S>
 void getrf_lookahead(LU<S>& f0) {
   hipStream_t s0 = f0.s, s1 = aux_stream();
   hipEvent_t e_upd = aux_event(0), e_panel = aux_event(1);
+  const long M = f0.M, N = f0.N, mn = std::min(M, N);
+  const long nleaf = (mn + PW - 1) / PW;
+  int* lists = static_cast<int*>(scratch(s0, kSlotLaLists, sizeof(int) * kLeafListInts * nleaf));
   LU<S> f1;
   init_lu(f1, s1, f0.M, f0.N, f0.A, f0.lda, f0.ipiv, f0.dinfo, f0.gemm_prec);
   f1.coop = ATLG_LA_COOP != 0;
-  const long M = f0.M, N = f0.N, mn = std::min(M, N), lda = f0.lda;
+  f0.leaf_lists = lists;
+  f1.leaf_lists = lists;
   // block 0 on the caller's stream
-  const long nb0 = std::min(kLaNB, mn);
   f0.win_lo = 0;
-  f0.win_hi = nb0;
-  getrf_rec(f0, 0, nb0);
-  if (N > nb0) {
-    laswp<S>(s0, N - nb0, at(f0, 0, nb0), lda, 0, (int)nb0, f0.ipiv, 1);
-    node_trsm(f0, 0, nb0, N - nb0);
-  }
-  for (long k = 0; k + kLaNB < mn; k += kLaNB) {
-    const long k1 = k + kLaNB, nb1 = std::min(kLaNB, mn - k1), mrem = M - k1;
-    const long rest = N - k1 - nb1;
-    // block k+1's columns first
-    node_update(f0, mrem, nb1, kLaNB, at(f0, k1, k), at(f0, k, k1), at(f0, k1, k1));
-    ATLG_HIP(hipEventRecord(e_upd, s0));
-    ATLG_HIP(hipStreamWaitEvent(s1, e_upd, 0));
-    f1.win_lo = k1;
-    f1.win_hi = k1 + nb1;
-    getrf_rec(f1, k1, nb1);
-    ATLG_HIP(hipEventRecord(e_panel, s1));
-    // the rest of step k's update, overlapping the panel: a capped grid
-    // leaves compute units to the panel's workgroups
-    if (rest > 0) {
-      f0.gemm_cap = ATLG_LA_GEMM_CAP;
-      node_update(f0, mrem, rest, kLaNB, at(f0, k1, k), at(f0, k, k1 + nb1),
-                  at(f0, k1, k1 + nb1));
-      f0.gemm_cap = 0;
+  f0.win_hi = std::min(kLaNB, mn);
+  getrf_rec(f0, 0, f0.win_hi);
+  for (long k0 = 0; k0 < mn; k0 += kLaNB) {
+    const long kb = std::min(kLaNB, mn - k0), k1 = k0 + kb;
+    const long nb1 = std::min(kLaNB, mn - k1);  // next block (none after the last)
+    if (nb1 > 0) {
+      apply_block_lists(f0, k0, kb, k1, nb1, 0, 0);
+      node_trsm_cols(f0, k0, kb, k1, nb1);
+      node_update(f0, M - k1, nb1, kb, at(f0, k1, k0), at(f0, k0, k1), at(f0, k1, k1));
+      ATLG_HIP(hipEventRecord(e_upd, s0));
+      ATLG_HIP(hipStreamWaitEvent(s1, e_upd, 0));
+      f1.win_lo = k1;
+      f1.win_hi = k1 + nb1;
+      getrf_rec(f1, k1, nb1);
+      ATLG_HIP(hipEventRecord(e_panel, s1));
     }
-    ATLG_HIP(hipStreamWaitEvent(s0, e_panel, 0));
+    const long r0 = k1 + nb1, rest = N - r0;
+    apply_block_lists(f0, k0, kb, 0, k0, r0, rest);
     if (rest > 0) {
-      laswp<S>(s0, rest, at(f0, 0, k1 + nb1), lda, (int)k1, (int)(k1 + nb1), f0.ipiv, 1);
-      node_trsm(f0, k1, nb1, rest);
+      node_trsm_cols(f0, k0, kb, r0, rest);
+      if (M > k1) {
+        f0.gemm_cap = ATLG_LA_GEMM_CAP;
+        node_update(f0, M - k1, rest, kb, at(f0, k1, k0), at(f0, k0, r0), at(f0, k1, r0));
+        f0.gemm_cap = 0;
+      }
     }
+    if (nb1 > 0) ATLG_HIP(hipStreamWaitEvent(s0, e_panel, 0));
   }
-  // interchanges of blocks 1.. reach the columns to their left
-  for (long k1 = kLaNB; k1 < mn; k1 += kLaNB)
-    laswp<S>(s0, k1, f0.A, lda, (int)k1, (int)std::min(mn, k1 + kLaNB), f0.ipiv, 1);
 }
 
 }  // namespace
