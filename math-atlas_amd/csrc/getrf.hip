@@ -94,20 +94,62 @@ __device__ inline bool better(double k1, int i1, double k2, int i2) {
   return k1 > k2 || (k1 == k2 && i1 < i2);
 }
 
-// wavefront argmax: after the call every lane holds the wave's best pair
-__device__ inline void wave_argmax(double& k, int& i) {
-#pragma unroll
-  for (int off = 32; off > 0; off >>= 1) {
-    const double k2 = __shfl_xor(k, off, 64);
-    const int i2 = __shfl_xor(i, off, 64);
-    if (better(k2, i2, k, i)) {
-      k = k2;
-      i = i2;
-    }
+// Cross-lane moves of the wavefront argmax: DPP within each 16-lane row
+// (quad_perm xor 1 and xor 2, then the half-row and row mirrors: after the
+// four steps every lane holds its row's best pair), v_readlane across the
+// four rows.  All 64 lanes must be active.
+template <int CTRL>
+__device__ __forceinline__ int dpp_i(int x) {
+  return __builtin_amdgcn_update_dpp(x, x, CTRL, 0xF, 0xF, false);
+}
+template <int CTRL>
+__device__ __forceinline__ double dpp_d(double x) {
+  const long long u = __double_as_longlong(x);
+  const int lo = dpp_i<CTRL>(static_cast<int>(u));
+  const int hi = dpp_i<CTRL>(static_cast<int>(u >> 32));
+  return __longlong_as_double((static_cast<long long>(hi) << 32) | static_cast<unsigned>(lo));
+}
+__device__ __forceinline__ double readlane_d(double x, int l) {
+  const long long u = __double_as_longlong(x);
+  const int lo = __builtin_amdgcn_readlane(static_cast<int>(u), l);
+  const int hi = __builtin_amdgcn_readlane(static_cast<int>(u >> 32), l);
+  return __longlong_as_double((static_cast<long long>(hi) << 32) | static_cast<unsigned>(lo));
+}
+template <int CTRL>
+__device__ __forceinline__ void argmax_step(double& k, int& i) {
+  const double k2 = dpp_d<CTRL>(k);
+  const int i2 = dpp_i<CTRL>(i);
+  if (better(k2, i2, k, i)) {
+    k = k2;
+    i = i2;
   }
 }
 
-// workgroup argmax (blockDim multiple of 64, <= 1024); result in every thread
+// wavefront argmax: after the call every lane holds the wave's best pair
+__device__ inline void wave_argmax(double& k, int& i) {
+  argmax_step<0xB1>(k, i);   // quad_perm [1,0,3,2]
+  argmax_step<0x4E>(k, i);   // quad_perm [2,3,0,1]
+  argmax_step<0x141>(k, i);  // row_half_mirror
+  argmax_step<0x140>(k, i);  // row_mirror
+  double kb = readlane_d(k, 0);
+  int ib = __builtin_amdgcn_readlane(i, 0);
+#pragma unroll
+  for (int r = 1; r < 4; r++) {
+    const double k2 = readlane_d(k, 16 * r);
+    const int i2 = __builtin_amdgcn_readlane(i, 16 * r);
+    if (better(k2, i2, kb, ib)) {
+      kb = k2;
+      ib = i2;
+    }
+  }
+  k = kb;
+  i = ib;
+}
+
+// workgroup argmax (blockDim multiple of 64, <= 1024); result in every
+// thread.  One barrier: every thread reduces the per-wave results itself.
+// sk/si are overwritten by the next call, so calls must be separated by
+// another barrier.
 __device__ inline void block_argmax(double& k, int& i, double* sk, int* si) {
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6, nw = blockDim.x >> 6;
   wave_argmax(k, i);
@@ -116,29 +158,96 @@ __device__ inline void block_argmax(double& k, int& i, double* sk, int* si) {
     si[w] = i;
   }
   __syncthreads();
-  if (w == 0) {
-    double kk = lane < nw ? sk[lane] : -3.0;
-    int ii = lane < nw ? si[lane] : INT_MAX;
-    wave_argmax(kk, ii);
-    if (lane == 0) {
-      sk[0] = kk;
-      si[0] = ii;
-    }
-  }
-  __syncthreads();
   k = sk[0];
   i = si[0];
-  __syncthreads();
+  for (int q = 1; q < nw; q++) {
+    const double k2 = sk[q];
+    const int i2 = si[q];
+    if (better(k2, i2, k, i)) {
+      k = k2;
+      i = i2;
+    }
+  }
 }
 
-// element j of a register row (j runtime, indices compile-time)
+// Element j of a register row (j runtime but uniform, indices
+// compile-time): a switch on the uniform j is a few scalar branches and one
+// move, where a select over all PW elements would cost PW vector selects.
+static_assert(PW == 32, "rget/rput list the 32 columns of a leaf");
 template <class S>
-__device__ __forceinline__ S rsel(const S (&r)[PW], int j) {
-  S x = S(0);
-#pragma unroll
-  for (int c = 0; c < PW; c++)
-    if (c == j) x = r[c];
-  return x;
+__device__ __forceinline__ S rget(const S (&r)[PW], int j) {
+  switch (j) {
+    case 0: return r[0];
+    case 1: return r[1];
+    case 2: return r[2];
+    case 3: return r[3];
+    case 4: return r[4];
+    case 5: return r[5];
+    case 6: return r[6];
+    case 7: return r[7];
+    case 8: return r[8];
+    case 9: return r[9];
+    case 10: return r[10];
+    case 11: return r[11];
+    case 12: return r[12];
+    case 13: return r[13];
+    case 14: return r[14];
+    case 15: return r[15];
+    case 16: return r[16];
+    case 17: return r[17];
+    case 18: return r[18];
+    case 19: return r[19];
+    case 20: return r[20];
+    case 21: return r[21];
+    case 22: return r[22];
+    case 23: return r[23];
+    case 24: return r[24];
+    case 25: return r[25];
+    case 26: return r[26];
+    case 27: return r[27];
+    case 28: return r[28];
+    case 29: return r[29];
+    case 30: return r[30];
+    case 31: return r[31];
+  }
+  return S(0);
+}
+template <class S>
+__device__ __forceinline__ void rput(S (&r)[PW], int j, S x) {
+  switch (j) {
+    case 0: r[0] = x; break;
+    case 1: r[1] = x; break;
+    case 2: r[2] = x; break;
+    case 3: r[3] = x; break;
+    case 4: r[4] = x; break;
+    case 5: r[5] = x; break;
+    case 6: r[6] = x; break;
+    case 7: r[7] = x; break;
+    case 8: r[8] = x; break;
+    case 9: r[9] = x; break;
+    case 10: r[10] = x; break;
+    case 11: r[11] = x; break;
+    case 12: r[12] = x; break;
+    case 13: r[13] = x; break;
+    case 14: r[14] = x; break;
+    case 15: r[15] = x; break;
+    case 16: r[16] = x; break;
+    case 17: r[17] = x; break;
+    case 18: r[18] = x; break;
+    case 19: r[19] = x; break;
+    case 20: r[20] = x; break;
+    case 21: r[21] = x; break;
+    case 22: r[22] = x; break;
+    case 23: r[23] = x; break;
+    case 24: r[24] = x; break;
+    case 25: r[25] = x; break;
+    case 26: r[26] = x; break;
+    case 27: r[27] = x; break;
+    case 28: r[28] = x; break;
+    case 29: r[29] = x; break;
+    case 30: r[30] = x; break;
+    case 31: r[31] = x; break;
+  }
 }
 
 // coherent (agent-scope) loads/stores for data shared between workgroups
@@ -344,6 +453,26 @@ __device__ inline typename Word<S>::U pub_bits(S x) {
   return u == Word<S>::kSent ? Word<S>::kQNaN : u;
 }
 
+// Panel element i of column c through a buffer resource on the panel: the
+// column offset travels in a scalar register, the thread's row offset in one
+// VGPR shared by all 32 columns, so loads and stores need no per-element
+// address registers (a store's address VGPRs could only be reused after the
+// store completes).  Usable while PW columns span less than 2 GiB.
+typedef unsigned int u32x2_t __attribute__((ext_vector_type(2)));
+constexpr int kBufDword3 = 0x00020000;  // raw buffer, 32-bit dwords (gfx9 family)
+__device__ inline double buf_ld(double, __amdgpu_buffer_rsrc_t r, unsigned vo, int so) {
+  return __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(r, vo, so, 0));
+}
+__device__ inline float buf_ld(float, __amdgpu_buffer_rsrc_t r, unsigned vo, int so) {
+  return __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(r, vo, so, 0));
+}
+__device__ inline void buf_st(double x, __amdgpu_buffer_rsrc_t r, unsigned vo, int so) {
+  __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(u32x2_t, x), r, vo, so, 0);
+}
+__device__ inline void buf_st(float x, __amdgpu_buffer_rsrc_t r, unsigned vo, int so) {
+  __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(unsigned, x), r, vo, so, 0);
+}
+
 // One leaf's exchange area: per column j and workgroup g the candidate
 // (key, row) and its row, plus row j.  Every slot is written at most once
 // per leaf, so a reader simply polls the slots it needs until none holds
@@ -405,25 +534,38 @@ panel_coop(int M, int w, int off, S* __restrict__ P, long lda, int* __restrict__
            int* __restrict__ list) {
   typedef typename Word<S>::U U;
   if (ATLG_PANEL_PRIO > 0) __builtin_amdgcn_s_setprio(ATLG_PANEL_PRIO);
-  __shared__ S prow[PW], jrow[PW];
+  __shared__ S jrow[PW];
   __shared__ S crows[CP_MAXG * PW];
+  __shared__ S crowm[CP_MAXG * PW];  // the same with columns <= j and >= w zeroed
   __shared__ S stage_c[PW], stage_j[PW];  // this workgroup's candidate row / row j
-  __shared__ int has_c, has_j, sbad;
+  __shared__ int sbad;
   __shared__ double sk[CP_THREADS / 64];
   __shared__ int si[CP_THREADS / 64];
   __shared__ int winner;
   __shared__ int pa[PW], pb[PW], cval[2 * PW], ccnt;
+  __shared__ int zcol;  // first column with a zero pivot (workgroup 0), -1: none
   const int t = threadIdx.x, G = gridDim.x, GT = G * CP_THREADS;
   const int g = blockIdx.x * CP_THREADS + t;
   Mailbox<S>* mb = &cb->mb[par];
   if (G > 1) mb_clear(&cb->mb[par ^ 1], gclear, g, GT);
-  if (t == 0) sbad = 0;
+  if (t == 0) {
+    sbad = 0;
+    zcol = -1;
+  }
+  const bool bufio = (long)PW * lda * (long)sizeof(S) < (1L << 31);
+  const __amdgpu_buffer_rsrc_t rs = __builtin_amdgcn_make_buffer_rsrc(P, 0, 0x7fffffff, kBufDword3);
+  const int cstride = static_cast<int>(lda * (long)sizeof(S));  // used only when bufio
   S v[RPT][PW];
 #pragma unroll
   for (int q = 0; q < RPT; q++) {
     const int i = g + q * GT;
+    const unsigned boff = static_cast<unsigned>(i) * sizeof(S);  // M <= 65536 rows
 #pragma unroll
-    for (int c = 0; c < PW; c++) v[q][c] = (i < M && c < w) ? P[i + (size_t)c * lda] : S(0);
+    for (int c = 0; c < PW; c++) {
+      S x = S(0);
+      if (i < M && c < w) x = bufio ? buf_ld(S(0), rs, boff, c * cstride) : P[i + (size_t)c * lda];
+      v[q][c] = x;
+    }
   }
   const int mn = min(M, w);
   bool ok = true;
@@ -434,7 +576,7 @@ panel_coop(int M, int w, int off, S* __restrict__ P, long lda, int* __restrict__
     for (int q = 0; q < RPT; q++) {
       const int i = g + q * GT;
       if (i >= j && i < M) {
-        const double kq = pkey(rsel(v[q], j));
+        const double kq = pkey(rget(v[q], j));
         if (better(kq, i, key, idx)) {
           key = kq;
           idx = i;
@@ -442,6 +584,7 @@ panel_coop(int M, int w, int off, S* __restrict__ P, long lda, int* __restrict__
       }
     }
     block_argmax(key, idx, sk, si);
+    int win = idx;  // the global pivot row (G == 1: this workgroup's)
     if (G == 1) {
       // the whole panel is in this workgroup: exchange through LDS only
 #pragma unroll
@@ -449,14 +592,16 @@ panel_coop(int M, int w, int off, S* __restrict__ P, long lda, int* __restrict__
         const int i = g + q * GT;
         if (i == idx) {
 #pragma unroll
-          for (int c = 0; c < PW; c++) crows[c] = (c < w) ? v[q][c] : S(0);
+          for (int c = 0; c < PW; c++) {
+            crows[c] = (c < w) ? v[q][c] : S(0);
+            crowm[c] = (c > j && c < w) ? v[q][c] : S(0);
+          }
         }
         if (i == j) {
 #pragma unroll
           for (int c = 0; c < PW; c++) jrow[c] = (c < w) ? v[q][c] : S(0);
         }
       }
-      if (t == 0) winner = idx;
       __syncthreads();
     } else {
       // Publish: the owners copy the candidate row and row j to LDS, then one
@@ -464,23 +609,18 @@ panel_coop(int M, int w, int off, S* __restrict__ P, long lda, int* __restrict__
       // the stores before it, so one lane storing a whole row would pay a
       // round trip per element).  A workgroup without rows >= j publishes
       // key -2 and a zero row (never chosen: rows j.. always exist).
-      if (t == 0) {
-        has_c = 0;
-        has_j = 0;
-      }
-      __syncthreads();
+      const bool has_c = idx != INT_MAX;
+      const bool has_j = (j % GT) / CP_THREADS == (int)blockIdx.x;
 #pragma unroll
       for (int q = 0; q < RPT; q++) {
         const int i = g + q * GT;
         if (i == idx) {
 #pragma unroll
           for (int c = 0; c < PW; c++) stage_c[c] = v[q][c];
-          has_c = 1;
         }
         if (i == j) {
 #pragma unroll
           for (int c = 0; c < PW; c++) stage_j[c] = v[q][c];
-          has_j = 1;
         }
       }
       __syncthreads();
@@ -543,7 +683,11 @@ panel_coop(int M, int w, int off, S* __restrict__ P, long lda, int* __restrict__
 #pragma unroll
       for (int u = 0; u < NU; u++) {
         const int e = t + u * CP_THREADS, c = e % PW;
-        if (e < G * PW) crows[e] = c < w ? Word<S>::val(cw[u]) : S(0);
+        if (e < G * PW) {
+          const S x = Word<S>::val(cw[u]);
+          crows[e] = c < w ? x : S(0);
+          crowm[e] = (c > j && c < w) ? x : S(0);
+        }
       }
       if (t >= 4 * PW && t < 5 * PW) jrow[t - 4 * PW] = want_j ? Word<S>::val(jw) : S(0);
       if (t < 64) {
@@ -557,19 +701,22 @@ panel_coop(int M, int w, int off, S* __restrict__ P, long lda, int* __restrict__
         ok = false;
         break;
       }
+      win = winner;
     }  // G > 1
     // rows j..M-1 always hold a candidate, so p is in [j, M); the clamp only
     // keeps a corrupted exchange from indexing outside the panel
-    const int p = (winner >= j && winner < M) ? winner : j;
-    const int bw = (p % GT) / CP_THREADS;  // workgroup holding row p
-    if (t < PW) prow[t] = crows[bw * PW + t];
-    __syncthreads();
+    const int p = (win >= j && win < M) ? win : j;
+    const int bw = (p % GT) / CP_THREADS;  // workgroup that published row p
+    const S* prow = crows + bw * PW;        // the pivot row
+    const S* pmsk = crowm + bw * PW;        // ... with columns <= j and >= w zeroed
     const S piv = prow[j];
+    // the pivots and the first zero pivot stay in LDS until the end: a
+    // global store here would stall this wave on its completion (stores
+    // and loads share one counter), delaying every workgroup's next column
     if (blockIdx.x == 0 && t == 0) {
-      ipiv[off + j] = off + p;
       pa[j] = off + j;
       pb[j] = off + p;
-      if (piv == S(0) && *dinfo == 0) *dinfo = off + j + 1;
+      if (piv == S(0) && zcol < 0) zcol = j;
     }
     if (piv != S(0)) {
       const bool recip = fabs(piv) >= std::numeric_limits<S>::min();
@@ -587,27 +734,46 @@ panel_coop(int M, int w, int off, S* __restrict__ P, long lda, int* __restrict__
           }
         }
         if (i > j && i < M) {
-          S l = S(0);
+          // scale column j, then the rank-1 update.  With a finite
+          // multiplier the masked pivot row makes the update one FMA per
+          // column (columns <= j and >= w see a zero); otherwise only the
+          // columns j < c < w are touched, as in the unmasked form.
+          const S x = rget(v[q], j);
+          S l;
+          if (recip) l = x * rinv;
+          else l = x / piv;
+          if (isfinite(l)) {
 #pragma unroll
-          for (int c = 0; c < PW; c++)  // scale column j (fused select keeps v in VGPRs)
-            if (c == j) {
-              v[q][c] = recip ? v[q][c] * rinv : v[q][c] / piv;
-              l = v[q][c];
-            }
+            for (int c = 0; c < PW; c++) v[q][c] -= l * pmsk[c];
+          } else {
 #pragma unroll
-          for (int c = 0; c < PW; c++)
-            if (c > j && c < w) v[q][c] -= l * prow[c];
+            for (int c = 0; c < PW; c++)
+              if (c > j && c < w) v[q][c] -= l * prow[c];
+          }
+          rput(v[q], j, l);
         }
       }
     }
   }
+  if (bufio && w == PW) {
 #pragma unroll
-  for (int q = 0; q < RPT; q++) {
-    const int i = g + q * GT;
-    if (i < M) {
+    for (int q = 0; q < RPT; q++) {
+      const int i = g + q * GT;
+      const unsigned boff = static_cast<unsigned>(i) * sizeof(S);
+      if (i < M) {
 #pragma unroll
-      for (int c = 0; c < PW; c++)
-        if (c < w) P[i + (size_t)c * lda] = v[q][c];
+        for (int c = 0; c < PW; c++) buf_st(v[q][c], rs, boff, c * cstride);
+      }
+    }
+  } else {
+#pragma unroll
+    for (int q = 0; q < RPT; q++) {
+      const int i = g + q * GT;
+      if (i < M) {
+#pragma unroll
+        for (int c = 0; c < PW; c++)
+          if (c < w) P[i + (size_t)c * lda] = v[q][c];
+      }
     }
   }
   if (!ok) {
@@ -617,6 +783,8 @@ panel_coop(int M, int w, int off, S* __restrict__ P, long lda, int* __restrict__
   }
   if (blockIdx.x == 0) {
     __syncthreads();
+    if (t < mn) ipiv[off + t] = pb[t];
+    if (t == 0 && zcol >= 0 && *dinfo == 0) *dinfo = off + zcol + 1;
     compose_swaps(pa, pb, mn, cval, &ccnt, list);
   }
 }
