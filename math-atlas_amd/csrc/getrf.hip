@@ -76,7 +76,10 @@ constexpr long kSpinLimit = 1L << 20;  // mailbox polls (each one round trip) be
 #ifndef ATLG_LA_GEMM_CAP
 #define ATLG_LA_GEMM_CAP 384
 #endif
-constexpr long kLaNB = 256;
+#ifndef ATLG_LA_NB
+#define ATLG_LA_NB 256
+#endif
+constexpr long kLaNB = ATLG_LA_NB;
 constexpr long kLaMin = 2048;
 
 // ---------------------------------------------------------------- helpers
@@ -892,6 +895,7 @@ struct LU {
   bool coop = true;     // cooperative launch of the panel kernel
   int gemm_cap = 0;     // grid cap of the trailing updates (0: none)
   int* leaf_lists = nullptr;  // look-ahead: leaf at column c keeps its list at c/PW
+  bool small_lds = false;     // runs beside a GEMM: keep kernels' LDS small
   int mb_par = 0;             // mailbox the next multi-workgroup panel uses
   int mb_g[2] = {0, 0};       // grid of each mailbox's last user (0: clean)
 };
@@ -994,6 +998,19 @@ void leaf(LU<S>& f, long off, long m, int w) {
 template <class S>
 void node_trsm_cols(LU<S>& f, long off, long nl, long c0, long nc) {
   if (nc <= 0) return;
+  if (f.small_lds && nl == kTrsmNB) {
+    // beside a running GEMM: two 64-column halves, whose diagonal-block
+    // inverses need a quarter of the LDS of a 128-block inverse (which
+    // would wait for a compute unit free of GEMM workgroups)
+    const long h = nl / 2;
+    trsm<S>(f.s, kLeft, kLower, kNoTrans, kUnit, h, nc, S(1), at(f, off, off), f.lda,
+            at(f, off, c0), f.lda);
+    gemm<S>(f.s, kNoTrans, kNoTrans, h, nc, h, S(-1), at(f, off + h, off), f.lda, at(f, off, c0),
+            f.lda, S(1), at(f, off + h, c0), f.lda);
+    trsm<S>(f.s, kLeft, kLower, kNoTrans, kUnit, h, nc, S(1), at(f, off + h, off + h), f.lda,
+            at(f, off + h, c0), f.lda);
+    return;
+  }
   if (nl >= kTrsmNB && off % kTrsmNB == 0 && nl % kTrsmNB == 0) {
     const long b0 = off / kTrsmNB, b1 = (off + nl) / kTrsmNB;
     long first = b0;
@@ -1146,6 +1163,7 @@ void getrf_lookahead(LU<S>& f0) {
   LU<S> f1;
   init_lu(f1, s1, f0.M, f0.N, f0.A, f0.lda, f0.ipiv, f0.dinfo, f0.gemm_prec);
   f1.coop = ATLG_LA_COOP != 0;
+  f1.small_lds = true;
   f0.leaf_lists = lists;
   f1.leaf_lists = lists;
   // block 0 on the caller's stream

@@ -62,14 +62,28 @@ potf2_inv_kernel(int n, double* __restrict__ A, long lda, int lower, int* __rest
   __shared__ double S[DB * LDS_LD];
   __shared__ int fail_sh;
   const int t = threadIdx.x;
-  for (int idx = t; idx < DB * DB; idx += P_THREADS) {
-    const int i = idx % DB, k = idx / DB;
-    double v = 0.0;
-    if (i >= k) {
-      if (i < n && k < n) v = lower ? A[i + (size_t)k * lda] : A[k + (size_t)i * lda];
-      else if (i == k) v = 1.0;
+  // fill in batches of 16 elements per thread, all loads of a batch in
+  // flight before the LDS stores
+  constexpr int PER = DB * DB / P_THREADS, BATCH = 16;
+  static_assert(PER % BATCH == 0, "fill batches");
+  for (int b0 = 0; b0 < PER; b0 += BATCH) {
+    double v[BATCH];
+#pragma unroll
+    for (int u = 0; u < BATCH; u++) {
+      const int idx = t + (b0 + u) * P_THREADS;
+      const int i = idx % DB, k = idx / DB;
+      double x = 0.0;
+      if (i >= k) {
+        if (i < n && k < n) x = lower ? A[i + (size_t)k * lda] : A[k + (size_t)i * lda];
+        else if (i == k) x = 1.0;
+      }
+      v[u] = x;
     }
-    S[i * LDS_LD + k] = v;
+#pragma unroll
+    for (int u = 0; u < BATCH; u++) {
+      const int idx = t + (b0 + u) * P_THREADS;
+      S[(idx % DB) * LDS_LD + idx / DB] = v[u];
+    }
   }
   if (t == 0) fail_sh = -1;
   __syncthreads();

@@ -54,19 +54,33 @@ trtri_diag_kernel(int na, const R* __restrict__ A, long lda, int trans, int lowe
   const long o = (long)b * NB;
   const int n = (int)min((long)NT, na - o);
   const bool tr = trans != 0;
-  for (int idx = t; idx < NT * NT; idx += TRI_THREADS) {
-    const int i = idx % NT, j = idx / NT;
-    R v = R(0);
-    if (i >= j) {
-      if (i < n && j < n) {
-        if (i == j && unit) v = R(1);
-        else v = lower ? opa(A, lda, tr, o + i, o + j)
-                       : opa(A, lda, tr, o + n - 1 - i, o + n - 1 - j);
-      } else if (i == j) {
-        v = R(1);
+  // the fill goes in batches of 16 elements per thread whose loads are all
+  // issued before any of them is stored to LDS (one memory latency per
+  // batch instead of one per element)
+  constexpr int PER = NT * NT / TRI_THREADS, BATCH = PER < 16 ? PER : 16;
+  for (int b0 = 0; b0 < PER; b0 += BATCH) {
+    R v[BATCH];
+#pragma unroll
+    for (int u = 0; u < BATCH; u++) {
+      const int idx = t + (b0 + u) * TRI_THREADS;
+      const int i = idx % NT, j = idx / NT;
+      R x = R(0);
+      if (i >= j) {
+        if (i < n && j < n) {
+          if (i == j && unit) x = R(1);
+          else x = lower ? opa(A, lda, tr, o + i, o + j)
+                         : opa(A, lda, tr, o + n - 1 - i, o + n - 1 - j);
+        } else if (i == j) {
+          x = R(1);
+        }
       }
+      v[u] = x;
     }
-    S[i * LD + j] = v;
+#pragma unroll
+    for (int u = 0; u < BATCH; u++) {
+      const int idx = t + (b0 + u) * TRI_THREADS;
+      S[(idx % NT) * LD + idx / NT] = v[u];
+    }
   }
   __syncthreads();
   lower_inverse_inplace<R, NT>(S, n);
