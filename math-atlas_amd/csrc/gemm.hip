@@ -24,6 +24,7 @@ namespace atlg {
 namespace {
 
 constexpr int BM = 128, BN = 128, BK = 16, NT = 256;
+static_assert(BM == BN, "triangular tile enumeration assumes square tiles");
 // Tile rows per group of the XCD-aware tile order, and whether workgroup ids
 // are regrouped per XCD at all (tuned constants).
 #ifndef ATLG_GEMM_GM
@@ -155,6 +156,39 @@ __device__ inline void tile_of(int bid, int nblk, int tilesM, int tilesN, int& t
   tn = in / gsz;
 }
 
+// Tiles of one triangle of a tilesM x tilesN tile grid (BM == BN): mode 1
+// keeps the tiles with tn <= tm (the lower triangle and everything below
+// it), mode 2 those with tm <= tn.  The kept tiles are enumerated row by row
+// (mode 1) or column by column (mode 2), so a triangular update launches
+// only the tiles it computes.
+__host__ __device__ inline int tri_tiles(int mode, int tilesM, int tilesN) {
+  const int R = mode == 1 ? tilesM : tilesN, Cn = mode == 1 ? tilesN : tilesM;
+  const int P = R < Cn ? R : Cn;
+  return P * (P + 1) / 2 + (R - P) * Cn;
+}
+__device__ inline void tri_tile_of(int id, int mode, int tilesM, int tilesN, int& tm, int& tn) {
+  const int R = mode == 1 ? tilesM : tilesN, Cn = mode == 1 ? tilesN : tilesM;
+  const int P = R < Cn ? R : Cn, T0 = P * (P + 1) / 2;
+  int a, b;  // a: index along R, b <= a along Cn
+  if (id < T0) {
+    a = (int)((sqrtf(8.0f * (float)id + 1.0f) - 1.0f) * 0.5f);
+    while ((a + 1) * (a + 2) / 2 <= id) a++;
+    while (a * (a + 1) / 2 > id) a--;
+    b = id - a * (a + 1) / 2;
+  } else {
+    const int r = id - T0;
+    a = P + r / Cn;
+    b = r - (r / Cn) * Cn;
+  }
+  if (mode == 1) {
+    tm = a;
+    tn = b;
+  } else {
+    tn = a;
+    tm = b;
+  }
+}
+
 template <class S, bool AT, bool BT, bool VA, bool VB>
 __global__ void __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(2, 2)))
 gemm_mfma(int M, int N, int K, S alpha, const S* A, long lda, const S* B, long ldb, S beta,
@@ -162,7 +196,7 @@ gemm_mfma(int M, int N, int K, S alpha, const S* A, long lda, const S* B, long l
   typedef Mfma<S> MF;
   if (skip && *skip) return;
   const int tilesM = (M + BM - 1) / BM, tilesN = (N + BN - 1) / BN;
-  const int ntiles = tilesM * tilesN;
+  const int ntiles = mode == 0 ? tilesM * tilesN : tri_tiles(mode, tilesM, tilesN);
 
   __shared__ __attribute__((aligned(16))) S As[2][BK * LDA_S];
   __shared__ __attribute__((aligned(16))) S Bs[2][BK * LDB_S];
@@ -171,10 +205,17 @@ gemm_mfma(int M, int N, int K, S alpha, const S* A, long lda, const S* B, long l
   // that a workgroup keeps its XCD) a strided loop over the tiles.
   for (int bid = blockIdx.x; bid < ntiles; bid += gridDim.x) {
   int tm, tn;
-  tile_of(bid, ntiles, tilesM, tilesN, tm, tn);
+  if (mode == 0) {
+    tile_of(bid, ntiles, tilesM, tilesN, tm, tn);
+  } else {
+    // the same XCD-compact renumbering, then the triangle's enumeration
+    const int per = ntiles / 8, rem = ntiles % 8, xcd = bid % 8, local = bid / 8;
+    const int id = !ATLG_GEMM_XCD ? bid
+                   : (xcd < rem) ? xcd * (per + 1) + local
+                                 : rem * (per + 1) + (xcd - rem) * per + local;
+    tri_tile_of(id, mode, tilesM, tilesN, tm, tn);
+  }
   const int m0 = tm * BM, n0 = tn * BN;
-  if (mode == 1 && n0 >= m0 + BM) continue;  // tile strictly above the diagonal
-  if (mode == 2 && m0 >= n0 + BN) continue;  // tile strictly below the diagonal
 
   const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
   const int wm0 = (wave >> 1) * WM, wn0 = (wave & 1) * WN;
@@ -320,7 +361,7 @@ void gemm(hipStream_t s, int TA, int TB, long M, long N, long K, S alpha, const 
     return;
   }
   const long tilesM = (M + BM - 1) / BM, tilesN = (N + BN - 1) / BN;
-  long nwg = tilesM * tilesN;
+  long nwg = mode == 0 ? tilesM * tilesN : tri_tiles(mode, (int)tilesM, (int)tilesN);
   if (max_wg > 0 && nwg > max_wg) nwg = std::max(8L, (long)(max_wg / 8) * 8);
   const dim3 g((unsigned)nwg);
   const bool at = is_trans(TA), bt = is_trans(TB);

@@ -46,13 +46,20 @@
 namespace atlg {
 namespace {
 
-constexpr int PW = 32;            // leaf panel width
+// Leaf panel width (32 or 64): wider leaves mean fewer recursion nodes
+// between leaves, at more registers and exchange volume per column.
+#ifndef ATLG_PANEL_PW
+#define ATLG_PANEL_PW 32
+#endif
+constexpr int PW = ATLG_PANEL_PW;  // leaf panel width
+static_assert(PW == 32 || PW == 64, "leaf panel width");
 constexpr int CP_THREADS = 256;   // workgroup of the cooperative panel
 constexpr int CP_MAXG = 64;       // workgroups of the cooperative panel
 constexpr int STEP_THREADS = 256;
 constexpr int KC = 256;           // interchanges composed per laswp chunk
 constexpr int CW = 16;            // columns per laswp workgroup
 constexpr int kLeafListInts = 1 + 4 * PW;  // one leaf's composed list: count + pairs
+constexpr int kJrowT = 128;  // first thread that gathers row j in the panel exchange
 constexpr long kSpinLimit = 1L << 20;  // mailbox polls (each one round trip) before giving up
 // Fewest panel rows per thread of the cooperative panel (1, 2 or 4): more
 // rows per thread means fewer workgroups exchanging candidates per column.
@@ -176,82 +183,154 @@ __device__ inline void block_argmax(double& k, int& i, double* sk, int* si) {
 // Element j of a register row (j runtime but uniform, indices
 // compile-time): a switch on the uniform j is a few scalar branches and one
 // move, where a select over all PW elements would cost PW vector selects.
-static_assert(PW == 32, "rget/rput list the 32 columns of a leaf");
+#define ATLG_RG(c) \
+  case c:          \
+    if constexpr (c < PW) return r[c]; else return S(0);
+#define ATLG_RP(c)                      \
+  case c:                               \
+    if constexpr (c < PW) r[c] = x;     \
+    break;
 template <class S>
 __device__ __forceinline__ S rget(const S (&r)[PW], int j) {
   switch (j) {
-    case 0: return r[0];
-    case 1: return r[1];
-    case 2: return r[2];
-    case 3: return r[3];
-    case 4: return r[4];
-    case 5: return r[5];
-    case 6: return r[6];
-    case 7: return r[7];
-    case 8: return r[8];
-    case 9: return r[9];
-    case 10: return r[10];
-    case 11: return r[11];
-    case 12: return r[12];
-    case 13: return r[13];
-    case 14: return r[14];
-    case 15: return r[15];
-    case 16: return r[16];
-    case 17: return r[17];
-    case 18: return r[18];
-    case 19: return r[19];
-    case 20: return r[20];
-    case 21: return r[21];
-    case 22: return r[22];
-    case 23: return r[23];
-    case 24: return r[24];
-    case 25: return r[25];
-    case 26: return r[26];
-    case 27: return r[27];
-    case 28: return r[28];
-    case 29: return r[29];
-    case 30: return r[30];
-    case 31: return r[31];
+    ATLG_RG(0)
+    ATLG_RG(1)
+    ATLG_RG(2)
+    ATLG_RG(3)
+    ATLG_RG(4)
+    ATLG_RG(5)
+    ATLG_RG(6)
+    ATLG_RG(7)
+    ATLG_RG(8)
+    ATLG_RG(9)
+    ATLG_RG(10)
+    ATLG_RG(11)
+    ATLG_RG(12)
+    ATLG_RG(13)
+    ATLG_RG(14)
+    ATLG_RG(15)
+    ATLG_RG(16)
+    ATLG_RG(17)
+    ATLG_RG(18)
+    ATLG_RG(19)
+    ATLG_RG(20)
+    ATLG_RG(21)
+    ATLG_RG(22)
+    ATLG_RG(23)
+    ATLG_RG(24)
+    ATLG_RG(25)
+    ATLG_RG(26)
+    ATLG_RG(27)
+    ATLG_RG(28)
+    ATLG_RG(29)
+    ATLG_RG(30)
+    ATLG_RG(31)
+    ATLG_RG(32)
+    ATLG_RG(33)
+    ATLG_RG(34)
+    ATLG_RG(35)
+    ATLG_RG(36)
+    ATLG_RG(37)
+    ATLG_RG(38)
+    ATLG_RG(39)
+    ATLG_RG(40)
+    ATLG_RG(41)
+    ATLG_RG(42)
+    ATLG_RG(43)
+    ATLG_RG(44)
+    ATLG_RG(45)
+    ATLG_RG(46)
+    ATLG_RG(47)
+    ATLG_RG(48)
+    ATLG_RG(49)
+    ATLG_RG(50)
+    ATLG_RG(51)
+    ATLG_RG(52)
+    ATLG_RG(53)
+    ATLG_RG(54)
+    ATLG_RG(55)
+    ATLG_RG(56)
+    ATLG_RG(57)
+    ATLG_RG(58)
+    ATLG_RG(59)
+    ATLG_RG(60)
+    ATLG_RG(61)
+    ATLG_RG(62)
+    ATLG_RG(63)
   }
   return S(0);
 }
 template <class S>
 __device__ __forceinline__ void rput(S (&r)[PW], int j, S x) {
   switch (j) {
-    case 0: r[0] = x; break;
-    case 1: r[1] = x; break;
-    case 2: r[2] = x; break;
-    case 3: r[3] = x; break;
-    case 4: r[4] = x; break;
-    case 5: r[5] = x; break;
-    case 6: r[6] = x; break;
-    case 7: r[7] = x; break;
-    case 8: r[8] = x; break;
-    case 9: r[9] = x; break;
-    case 10: r[10] = x; break;
-    case 11: r[11] = x; break;
-    case 12: r[12] = x; break;
-    case 13: r[13] = x; break;
-    case 14: r[14] = x; break;
-    case 15: r[15] = x; break;
-    case 16: r[16] = x; break;
-    case 17: r[17] = x; break;
-    case 18: r[18] = x; break;
-    case 19: r[19] = x; break;
-    case 20: r[20] = x; break;
-    case 21: r[21] = x; break;
-    case 22: r[22] = x; break;
-    case 23: r[23] = x; break;
-    case 24: r[24] = x; break;
-    case 25: r[25] = x; break;
-    case 26: r[26] = x; break;
-    case 27: r[27] = x; break;
-    case 28: r[28] = x; break;
-    case 29: r[29] = x; break;
-    case 30: r[30] = x; break;
-    case 31: r[31] = x; break;
+    ATLG_RP(0)
+    ATLG_RP(1)
+    ATLG_RP(2)
+    ATLG_RP(3)
+    ATLG_RP(4)
+    ATLG_RP(5)
+    ATLG_RP(6)
+    ATLG_RP(7)
+    ATLG_RP(8)
+    ATLG_RP(9)
+    ATLG_RP(10)
+    ATLG_RP(11)
+    ATLG_RP(12)
+    ATLG_RP(13)
+    ATLG_RP(14)
+    ATLG_RP(15)
+    ATLG_RP(16)
+    ATLG_RP(17)
+    ATLG_RP(18)
+    ATLG_RP(19)
+    ATLG_RP(20)
+    ATLG_RP(21)
+    ATLG_RP(22)
+    ATLG_RP(23)
+    ATLG_RP(24)
+    ATLG_RP(25)
+    ATLG_RP(26)
+    ATLG_RP(27)
+    ATLG_RP(28)
+    ATLG_RP(29)
+    ATLG_RP(30)
+    ATLG_RP(31)
+    ATLG_RP(32)
+    ATLG_RP(33)
+    ATLG_RP(34)
+    ATLG_RP(35)
+    ATLG_RP(36)
+    ATLG_RP(37)
+    ATLG_RP(38)
+    ATLG_RP(39)
+    ATLG_RP(40)
+    ATLG_RP(41)
+    ATLG_RP(42)
+    ATLG_RP(43)
+    ATLG_RP(44)
+    ATLG_RP(45)
+    ATLG_RP(46)
+    ATLG_RP(47)
+    ATLG_RP(48)
+    ATLG_RP(49)
+    ATLG_RP(50)
+    ATLG_RP(51)
+    ATLG_RP(52)
+    ATLG_RP(53)
+    ATLG_RP(54)
+    ATLG_RP(55)
+    ATLG_RP(56)
+    ATLG_RP(57)
+    ATLG_RP(58)
+    ATLG_RP(59)
+    ATLG_RP(60)
+    ATLG_RP(61)
+    ATLG_RP(62)
+    ATLG_RP(63)
   }
 }
+#undef ATLG_RG
+#undef ATLG_RP
 
 // coherent (agent-scope) loads/stores for data shared between workgroups
 __device__ inline double ld_agent(const double* p) {
@@ -647,7 +726,7 @@ panel_coop(int M, int w, int off, S* __restrict__ P, long lda, int* __restrict__
       unsigned long long kw = kKeySent;
       unsigned iw = kIdxSent;
       U jw = Word<S>::kSent;
-      const bool want_k = t < G, want_j = t >= 4 * PW && t - 4 * PW < w;
+      const bool want_k = t < G, want_j = t >= kJrowT && t - kJrowT < w;
       long spins = 0;
       for (;;) {
         // issue every pending load before looking at any result, so that
@@ -667,7 +746,7 @@ panel_coop(int M, int w, int off, S* __restrict__ P, long lda, int* __restrict__
         }
         if (pk) kw = ld_bits(&mb->key[j][t]);
         if (pi) iw = ld_bits(&mb->idx[j][t]);
-        if (pj) jw = ld_bits(&mb->jrow[j][t - 4 * PW]);
+        if (pj) jw = ld_bits(&mb->jrow[j][t - kJrowT]);
         bool waiting = (want_k && (kw == kKeySent || iw == kIdxSent)) ||
                        (want_j && jw == Word<S>::kSent);
 #pragma unroll
@@ -692,7 +771,7 @@ panel_coop(int M, int w, int off, S* __restrict__ P, long lda, int* __restrict__
           crowm[e] = (c > j && c < w) ? x : S(0);
         }
       }
-      if (t >= 4 * PW && t < 5 * PW) jrow[t - 4 * PW] = want_j ? Word<S>::val(jw) : S(0);
+      if (t >= kJrowT && t < kJrowT + PW) jrow[t - kJrowT] = want_j ? Word<S>::val(jw) : S(0);
       if (t < 64) {
         double kk = want_k ? Word<double>::val(kw) : -3.0;
         int ii = want_k ? static_cast<int>(iw) : INT_MAX;

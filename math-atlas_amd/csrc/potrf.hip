@@ -37,6 +37,10 @@ __device__ __forceinline__ double lane_value(double v, int src) {
 }
 
 constexpr int SB = 32;  // sub-block order of the in-workgroup factorization
+#ifndef ATLG_POTRF_RB
+#define ATLG_POTRF_RB 256
+#endif
+constexpr long RB = ATLG_POTRF_RB;  // block width of the right-looking outer steps
 
 // Factors the n x n (n <= DB) diagonal block at A and writes inv(L) of the
 // block to Dinv (DB x DB, column-major, zeros above the diagonal).
@@ -201,7 +205,10 @@ void potrf_rec(hipStream_t s, bool lower, long N, double* A, long lda, int* dinf
     ATLG_LAUNCHED();
     return;
   }
-  long nl = ((N / 2) / DB) * DB;
+  // Right-looking by RB-column blocks (then ATLAS's halving inside a block):
+  // each step's update is one large SYRK instead of the many small GEMMs of
+  // a halving recursion over the whole order.
+  long nl = (N > 2 * RB) ? RB : ((N / 2) / DB) * DB;
   if (nl < DB) nl = DB;
   const long nr = N - nl;
   potrf_rec(s, lower, nl, A, lda, dinfo, off, W);
@@ -222,6 +229,74 @@ void potrf_rec(hipStream_t s, bool lower, long N, double* A, long lda, int* dinf
   potrf_rec(s, lower, nr, A22, lda, dinfo, off + (int)nl, W);
 }
 
+// Look-ahead (1) or plain right-looking steps (0).  Measured at N = 16384
+// both take the same time: the diagonal-block chain (two 128-block leaf
+// kernels and the panel solve per step) is the longer of the two paths, so
+// overlapping the trailing SYRK with it gains nothing.
+// Grid cap of the look-ahead's trailing SYRK (workgroups).
+#ifndef ATLG_CHOL_LOOKAHEAD
+#define ATLG_CHOL_LOOKAHEAD 0
+#endif
+#ifndef ATLG_CHOL_CAP
+#define ATLG_CHOL_CAP 384
+#endif
+constexpr long kCholLaMin = 2048;
+
+// Right-looking Cholesky by RB-column blocks with one step of look-ahead:
+// while the caller's stream applies step k's SYRK to the columns right of
+// block k+1, the auxiliary stream factors block k+1 (updated first) and
+// solves its panel.  lower: A = L L^T, panel L21 = A21 L11^-T, update
+// A22 -= L21 L21^T; upper: A = U^T U, U12 = U11^-T A12, A22 -= U12^T U12.
+void potrf_lookahead(hipStream_t s0, bool lower, long N, double* A, long lda, int* dinfo,
+                     double* W) {
+  hipStream_t s1 = aux_stream();
+  hipEvent_t e_upd = aux_event(0), e_panel = aux_event(1);
+  auto at = [&](long i, long j) { return A + i + j * lda; };
+  // the panel of the block at k (kb columns) below/right of it, rows/cols r
+  auto panel_ptr = [&](long k, long kb) { return lower ? at(k + kb, k) : at(k, k + kb); };
+  auto solve_panel = [&](hipStream_t s, long k, long kb, long r) {
+    if (r <= 0) return;
+    const double* Winv = W + (size_t)(k / DB) * DB * DB;
+    if (lower)
+      trsm(s, kRight, kLower, kTrans, kNonUnit, r, kb, 1.0, at(k, k), lda, panel_ptr(k, kb), lda,
+           dinfo, Winv, 1);
+    else
+      trsm(s, kLeft, kUpper, kTrans, kNonUnit, kb, r, 1.0, at(k, k), lda, panel_ptr(k, kb), lda,
+           dinfo, Winv, 0);
+  };
+  // A(c0.., c0..c0+nc) -= P_c P_c^T restricted to the stored triangle, P the
+  // panel of block k (kb wide) from its row/column c0 on; m rows/cols in all
+  auto update = [&](hipStream_t s, long k, long kb, long c0, long m, long nc, int cap) {
+    if (m <= 0 || nc <= 0) return;
+    if (lower) {
+      const double* P = at(c0, k);
+      gemm(s, kNoTrans, kTrans, m, nc, kb, -1.0, P, lda, P, lda, 1.0, at(c0, c0), lda, 1, dinfo,
+           cap);
+    } else {
+      const double* P = at(k, c0);
+      gemm(s, kTrans, kNoTrans, nc, m, kb, -1.0, P, lda, P, lda, 1.0, at(c0, c0), lda, 2, dinfo,
+           cap);
+    }
+  };
+  const long kb0 = std::min(RB, N);
+  potrf_rec(s0, lower, kb0, A, lda, dinfo, 0, W);
+  solve_panel(s0, 0, kb0, N - kb0);
+  for (long k0 = 0; k0 + kb0 < N; k0 += RB) {
+    const long kb = std::min(RB, N - k0), k1 = k0 + kb, nb1 = std::min(RB, N - k1);
+    const long rest = N - k1 - nb1;
+    // block k+1 first (all its rows), then hand it to the auxiliary stream
+    update(s0, k0, kb, k1, N - k1, nb1, 0);
+    ATLG_HIP(hipEventRecord(e_upd, s0));
+    ATLG_HIP(hipStreamWaitEvent(s1, e_upd, 0));
+    potrf_rec(s1, lower, nb1, at(k1, k1), lda, dinfo, (int)k1, W);
+    solve_panel(s1, k1, nb1, rest);
+    ATLG_HIP(hipEventRecord(e_panel, s1));
+    // the rest of step k's update beside it
+    update(s0, k0, kb, k1 + nb1, rest, rest, ATLG_CHOL_CAP);
+    ATLG_HIP(hipStreamWaitEvent(s0, e_panel, 0));
+  }
+}
+
 }  // namespace
 
 void potrf(hipStream_t s, int uplo, long N, double* A, long lda, int* dinfo) {
@@ -229,7 +304,10 @@ void potrf(hipStream_t s, int uplo, long N, double* A, long lda, int* dinfo) {
   if (N <= 0) return;
   const long nblk = (N + DB - 1) / DB;
   double* W = static_cast<double*>(scratch(s, kSlotFactInv, sizeof(double) * nblk * DB * DB));
-  potrf_rec(s, uplo == kLower, N, A, lda, dinfo, 0, W);
+  if (ATLG_CHOL_LOOKAHEAD && N >= kCholLaMin)
+    potrf_lookahead(s, uplo == kLower, N, A, lda, dinfo, W);
+  else
+    potrf_rec(s, uplo == kLower, N, A, lda, dinfo, 0, W);
 }
 
 void potrs(hipStream_t s, int uplo, long N, long NRHS, const double* A, long lda, double* B,
