@@ -54,7 +54,13 @@ namespace {
 constexpr int PW = ATLG_PANEL_PW;  // leaf panel width
 static_assert(PW == 32 || PW == 64, "leaf panel width");
 constexpr int CP_THREADS = 256;   // workgroup of the cooperative panel
-constexpr int CP_MAXG = 64;       // workgroups of the cooperative panel
+// Most workgroups of the cooperative panel (64 or 128): one panel row per
+// thread up to CP_MAXG * 256 rows, more rows per thread beyond.
+#ifndef ATLG_PANEL_MAXG
+#define ATLG_PANEL_MAXG 64
+#endif
+constexpr int CP_MAXG = ATLG_PANEL_MAXG;
+static_assert(CP_MAXG == 64 || CP_MAXG == 128, "panel workgroups");
 constexpr int STEP_THREADS = 256;
 constexpr int KC = 256;           // interchanges composed per laswp chunk
 constexpr int CW = 16;            // columns per laswp workgroup
@@ -79,12 +85,18 @@ constexpr long kSpinLimit = 1L << 20;  // mailbox polls (each one round trip) be
 #ifndef ATLG_PANEL_PRIO
 #define ATLG_PANEL_PRIO 3
 #endif
-// Grid cap of the look-ahead trailing update (workgroups; 0 = no cap).
+// Grid cap of the look-ahead trailing update (workgroups; 0 = no cap),
+// applied while the update is smaller than ATLG_LA_CAP_AREA elements: then
+// the panel beside it is the longer path and keeps compute units; a larger
+// update is the longer path itself and gets the whole device.
 #ifndef ATLG_LA_GEMM_CAP
 #define ATLG_LA_GEMM_CAP 384
 #endif
+#ifndef ATLG_LA_CAP_AREA
+#define ATLG_LA_CAP_AREA (16384L * 16384L)
+#endif
 #ifndef ATLG_LA_NB
-#define ATLG_LA_NB 256
+#define ATLG_LA_NB 512
 #endif
 constexpr long kLaNB = ATLG_LA_NB;
 constexpr long kLaMin = 2048;
@@ -623,7 +635,8 @@ panel_coop(int M, int w, int off, S* __restrict__ P, long lda, int* __restrict__
   __shared__ int sbad;
   __shared__ double sk[CP_THREADS / 64];
   __shared__ int si[CP_THREADS / 64];
-  __shared__ int winner;
+  __shared__ double gk[2];  // best candidate of workgroups 0..63, 64..127
+  __shared__ int gi[2];
   __shared__ int pa[PW], pb[PW], cval[2 * PW], ccnt;
   __shared__ int zcol;  // first column with a zero pivot (workgroup 0), -1: none
   const int t = threadIdx.x, G = gridDim.x, GT = G * CP_THREADS;
@@ -772,18 +785,21 @@ panel_coop(int M, int w, int off, S* __restrict__ P, long lda, int* __restrict__
         }
       }
       if (t >= kJrowT && t < kJrowT + PW) jrow[t - kJrowT] = want_j ? Word<S>::val(jw) : S(0);
-      if (t < 64) {
+      if (t < 128) {  // the candidates of workgroups 0..63 and 64..127
         double kk = want_k ? Word<double>::val(kw) : -3.0;
         int ii = want_k ? static_cast<int>(iw) : INT_MAX;
         wave_argmax(kk, ii);
-        if (t == 0) winner = ii;
+        if ((t & 63) == 0) {
+          gk[t >> 6] = kk;
+          gi[t >> 6] = ii;
+        }
       }
       __syncthreads();
       if (sbad) {
         ok = false;
         break;
       }
-      win = winner;
+      win = better(gk[1], gi[1], gk[0], gi[0]) ? gi[1] : gi[0];
     }  // G > 1
     // rows j..M-1 always hold a candidate, so p is in [j, M); the clamp only
     // keeps a corrupted exchange from indexing outside the panel
@@ -1268,7 +1284,7 @@ void getrf_lookahead(LU<S>& f0) {
     if (rest > 0) {
       node_trsm_cols(f0, k0, kb, r0, rest);
       if (M > k1) {
-        f0.gemm_cap = ATLG_LA_GEMM_CAP;
+        f0.gemm_cap = (M - k1) * rest < ATLG_LA_CAP_AREA ? ATLG_LA_GEMM_CAP : 0;
         node_update(f0, M - k1, rest, kb, at(f0, k1, k0), at(f0, k0, r0), at(f0, k1, r0));
         f0.gemm_cap = 0;
       }
