@@ -55,7 +55,12 @@ hipStream_t library_stream();
 // priority) and two events, for work that overlaps the caller's stream
 // (getrf's look-ahead panel).  Created on first use.
 hipStream_t aux_stream();
-hipEvent_t aux_event(int i);
+hipEvent_t aux_event(int i);  // i = 0..3
+// A pair of streams restricted to disjoint compute units: *small gets the
+// first `cus` units, *large the rest (hipExtStreamCreateWithCUMask).  Made
+// once per device, on the first call's `cus`; false when cus <= 0 or the
+// masked streams are unavailable.
+bool split_streams(int cus, hipStream_t* small, hipStream_t* large);
 
 // Per-stream scratch memory in named slots.  A slot grows (after a stream
 // synchronize) and is reused by later calls on the same stream; pointers from

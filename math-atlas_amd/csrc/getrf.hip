@@ -85,10 +85,18 @@ constexpr long kSpinLimit = 1L << 20;  // mailbox polls (each one round trip) be
 #ifndef ATLG_PANEL_PRIO
 #define ATLG_PANEL_PRIO 3
 #endif
-// Grid cap of the look-ahead trailing update (workgroups; 0 = no cap),
-// applied while the update is smaller than ATLG_LA_CAP_AREA elements: then
-// the panel beside it is the longer path and keeps compute units; a larger
-// update is the longer path itself and gets the whole device.
+// Compute units reserved for the look-ahead panels (0: panels and updates
+// share the device).  64 units give a 64-workgroup panel grid one unit per
+// workgroup; a grid that cannot all be resident ends at the mailbox spin
+// limit with an error rather than waiting.
+#ifndef ATLG_LA_PANEL_CUS
+#define ATLG_LA_PANEL_CUS 64
+#endif
+// Without reserved units: grid cap of the look-ahead trailing update
+// (workgroups; 0 = no cap), applied while the update is smaller than
+// ATLG_LA_CAP_AREA elements: then the panel beside it is the longer path and
+// keeps compute units; a larger update is the longer path itself and gets
+// the whole device.
 #ifndef ATLG_LA_GEMM_CAP
 #define ATLG_LA_GEMM_CAP 384
 #endif
@@ -1252,6 +1260,20 @@ template <class S>
 void getrf_lookahead(LU<S>& f0) {
   hipStream_t s0 = f0.s, s1 = aux_stream();
   hipEvent_t e_upd = aux_event(0), e_panel = aux_event(1);
+  // With ATLG_LA_PANEL_CUS > 0 the panels run on a stream confined to that
+  // many compute units and everything else on a stream confined to the
+  // rest, so the latency-bound panel never shares a SIMD with the GEMM; the
+  // caller's stream forks to the pair and joins at the end.
+  hipStream_t caller = s0, sp = nullptr, sg = nullptr;
+  const bool split = ATLG_LA_PANEL_CUS > 0 && split_streams(ATLG_LA_PANEL_CUS, &sp, &sg);
+  if (split) {
+    ATLG_HIP(hipEventRecord(aux_event(2), caller));
+    ATLG_HIP(hipStreamWaitEvent(sg, aux_event(2), 0));
+    ATLG_HIP(hipStreamWaitEvent(sp, aux_event(2), 0));
+    s0 = sg;
+    s1 = sp;
+    f0.s = sg;
+  }
   const long M = f0.M, N = f0.N, mn = std::min(M, N);
   const long nleaf = (mn + PW - 1) / PW;
   int* lists = static_cast<int*>(scratch(s0, kSlotLaLists, sizeof(int) * kLeafListInts * nleaf));
@@ -1284,12 +1306,17 @@ void getrf_lookahead(LU<S>& f0) {
     if (rest > 0) {
       node_trsm_cols(f0, k0, kb, r0, rest);
       if (M > k1) {
-        f0.gemm_cap = (M - k1) * rest < ATLG_LA_CAP_AREA ? ATLG_LA_GEMM_CAP : 0;
+        f0.gemm_cap = !split && (M - k1) * rest < ATLG_LA_CAP_AREA ? ATLG_LA_GEMM_CAP : 0;
         node_update(f0, M - k1, rest, kb, at(f0, k1, k0), at(f0, k0, r0), at(f0, k1, r0));
         f0.gemm_cap = 0;
       }
     }
     if (nb1 > 0) ATLG_HIP(hipStreamWaitEvent(s0, e_panel, 0));
+  }
+  if (split) {  // every panel was joined into s0 above
+    ATLG_HIP(hipEventRecord(aux_event(3), s0));
+    ATLG_HIP(hipStreamWaitEvent(caller, aux_event(3), 0));
+    f0.s = caller;
   }
 }
 

@@ -11,6 +11,7 @@
 #include <cstdio>
 #include <cstring>
 #include <map>
+#include <vector>
 #include <mutex>
 #include <string>
 #include <utility>
@@ -50,7 +51,9 @@ hipStream_t library_stream() {
 namespace {
 struct Aux {
   hipStream_t stream = nullptr;
-  hipEvent_t ev[2] = {nullptr, nullptr};
+  hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
+  hipStream_t split[2] = {nullptr, nullptr};  // CU-masked pair, created on first use
+  int split_cus = -1;
 };
 std::map<int, Aux> g_aux;
 Aux& aux_for_device() {
@@ -67,7 +70,36 @@ Aux& aux_for_device() {
 }  // namespace
 
 hipStream_t aux_stream() { return aux_for_device().stream; }
-hipEvent_t aux_event(int i) { return aux_for_device().ev[i & 1]; }
+hipEvent_t aux_event(int i) { return aux_for_device().ev[i & 3]; }
+
+bool split_streams(int cus, hipStream_t* small, hipStream_t* large) {
+  Aux& a = aux_for_device();
+  std::lock_guard<std::mutex> lk(g_mu);
+  if (a.split_cus < 0) {
+    int dev = 0, total = 0;
+    ATLG_HIP(hipGetDevice(&dev));
+    ATLG_HIP(hipDeviceGetAttribute(&total, hipDeviceAttributeMultiprocessorCount, dev));
+    a.split_cus = 0;
+    if (cus > 0 && cus < total) {
+      const int words = (total + 31) / 32;
+      std::vector<uint32_t> m0(words, 0u), m1(words, 0u);
+      for (int c = 0; c < total; c++) (c < cus ? m0 : m1)[c / 32] |= 1u << (c % 32);
+      if (hipExtStreamCreateWithCUMask(&a.split[0], words, m0.data()) == hipSuccess &&
+          hipExtStreamCreateWithCUMask(&a.split[1], words, m1.data()) == hipSuccess) {
+        a.split_cus = cus;
+      } else {
+        (void)hipGetLastError();
+        for (auto& st : a.split)
+          if (st) (void)hipStreamDestroy(st);
+        a.split[0] = a.split[1] = nullptr;
+      }
+    }
+  }
+  if (a.split_cus <= 0) return false;
+  *small = a.split[0];
+  *large = a.split[1];
+  return true;
+}
 
 void* scratch(hipStream_t s, Slot slot, size_t bytes) {
   if (bytes == 0) bytes = 256;
