@@ -40,8 +40,9 @@ constexpr int KPAD = 64;     // K of the converted copies is a multiple of this
 
 // Register layout of v_mfma_f32_16x16x32_bf16: lane l supplies
 // X1[l&15][8(l>>4) .. 8(l>>4)+7] and X2[8(l>>4) .. +7][l&15]; D[i][j] is in
-// lane j + 16*(i>>2), register i&3.  The kernel passes the B tile as X1 and
-// the A tile as X2, so lane l, register r carries C[m = l&15][n = 4(l>>4)+r].
+// lane j + 16*(i>>2), register i&3.  The kernel passes the A tile as X1 and
+// the B tile as X2, so lane l, register r carries C[m = 4(l>>4)+r][n = l&15]:
+// four consecutive rows of one column, one 16-byte access of column-major C.
 
 // dst[x][k] (k contiguous, pitch ldd, x < X, k < Kp) = bf16(src(x, k)) where
 // src(x, k) = S[x + k*lds] (TRANS: x contiguous in the source) or
@@ -105,7 +106,9 @@ to_bf16_kernel(long X, long K, long Kp, const float* __restrict__ S, long lds,
 
 // C[m][n] = alpha * sum_k At[m][k] * Bt[n][k] + beta * C[m][n]; with
 // NPROD = 6 each operand comes as hi/mid/lo copies (A0/A1/A2, B0/B1/B2).
-template <int NPROD>
+// VEC: C is 16-byte aligned and ldc a multiple of 4 (vector C accesses).
+// The C tile is read before the K loop, so its latency overlaps the MFMAs.
+template <int NPROD, bool VEC>
 __global__ void __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(1, 2)))
 gemm_bf16_kernel(int M, int N, int Kp, float alpha, const __bf16* __restrict__ A0,
                  const __bf16* __restrict__ A1, const __bf16* __restrict__ A2, long lda,
@@ -152,11 +155,26 @@ gemm_bf16_kernel(int M, int N, int Kp, float alpha, const __bf16* __restrict__ A
       }
   };
 
-  f4 acc[TM][TN];
+  f4 acc[TM][TN], cin[TM][TN];
+  const int cm = 4 * (lane >> 4), cn = lane & 15;  // this lane's C rows cm..cm+3, column cn
 #pragma unroll
   for (int i = 0; i < TM; i++)
 #pragma unroll
-    for (int j = 0; j < TN; j++) acc[i][j] = f4{0, 0, 0, 0};
+    for (int j = 0; j < TN; j++) {
+      acc[i][j] = f4{0, 0, 0, 0};
+      cin[i][j] = f4{0, 0, 0, 0};
+      if (beta != 0.0f) {
+        const int gm = m0 + wm0 + i * 16 + cm, gn = n0 + wn0 + j * 16 + cn;
+        const float* c = C + (size_t)gm + (size_t)gn * ldc;
+        if (VEC && gm + 3 < M && gn < N) {
+          cin[i][j] = *reinterpret_cast<const f4*>(c);
+        } else if (gn < N) {
+#pragma unroll
+          for (int r = 0; r < 4; r++)
+            if (gm + r < M) cin[i][j][r] = c[r];
+        }
+      }
+    }
 
   const int nk = Kp / BK;
   gload(0);
@@ -181,19 +199,18 @@ gemm_bf16_kernel(int M, int N, int Kp, float alpha, const __bf16* __restrict__ A
           bfr[o][j] =
               *reinterpret_cast<const bf16x8*>(&Bs[cur][o][(wn0 + j * 16 + fx) * LDK + ks + fk]);
       }
+      // Products in the order small terms first: hi*lo, lo*hi, mid*mid,
+      // hi*mid, mid*hi, hi*hi (only hi*hi without the split).  The product
+      // loop is outermost so consecutive MFMAs go to different accumulators.
+      constexpr int kPA[6] = {0, 2, 1, 0, 1, 0}, kPB[6] = {2, 0, 1, 1, 0, 0};
 #pragma unroll
-      for (int i = 0; i < TM; i++)
+      for (int p = 6 - NPROD; p < 6; p++)
 #pragma unroll
-        for (int j = 0; j < TN; j++) {
-          if (NPROD == 6) {  // small terms first: lo*hi, hi*lo, mid*mid, mid*hi, hi*mid
-            acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(bfr[2][j], af[0][i], acc[i][j], 0, 0, 0);
-            acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(bfr[0][j], af[2][i], acc[i][j], 0, 0, 0);
-            acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(bfr[1][j], af[1][i], acc[i][j], 0, 0, 0);
-            acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(bfr[1][j], af[0][i], acc[i][j], 0, 0, 0);
-            acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(bfr[0][j], af[1][i], acc[i][j], 0, 0, 0);
-          }
-          acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(bfr[0][j], af[0][i], acc[i][j], 0, 0, 0);
-        }
+        for (int i = 0; i < TM; i++)
+#pragma unroll
+          for (int j = 0; j < TN; j++)
+            acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(af[kPA[p]][i], bfr[kPB[p]][j],
+                                                                acc[i][j], 0, 0, 0);
     }
     if (more) lstore(cur ^ 1);
     __syncthreads();
@@ -201,17 +218,21 @@ gemm_bf16_kernel(int M, int N, int Kp, float alpha, const __bf16* __restrict__ A
 #pragma unroll
   for (int i = 0; i < TM; i++)
 #pragma unroll
-    for (int j = 0; j < TN; j++)
+    for (int j = 0; j < TN; j++) {
+      const int gm = m0 + wm0 + i * 16 + cm, gn = n0 + wn0 + j * 16 + cn;
+      float* c = C + (size_t)gm + (size_t)gn * ldc;
+      f4 v;
 #pragma unroll
-      for (int r = 0; r < 4; r++) {
-        const int gm = m0 + wm0 + i * 16 + (lane & 15);
-        const int gn = n0 + wn0 + j * 16 + 4 * (lane >> 4) + r;
-        if (gm < M && gn < N) {
-          float* c = C + (size_t)gm + (size_t)gn * ldc;
-          const float v = alpha * acc[i][j][r];
-          *c = (beta == 0.0f) ? v : v + beta * (*c);
-        }
+      for (int r = 0; r < 4; r++)
+        v[r] = (beta == 0.0f) ? alpha * acc[i][j][r] : alpha * acc[i][j][r] + beta * cin[i][j][r];
+      if (VEC && gm + 3 < M && gn < N) {
+        *reinterpret_cast<f4*>(c) = v;
+      } else if (gn < N) {
+#pragma unroll
+        for (int r = 0; r < 4; r++)
+          if (gm + r < M) c[r] = v[r];
       }
+    }
 }
 
 }  // namespace
@@ -242,14 +263,12 @@ void gemm_bf16(hipStream_t s, long M, long N, long K, float alpha, const float* 
                      Bc[0], Kp, split ? Bc[1] : nullptr, split ? Bc[2] : nullptr);
   ATLG_LAUNCHED();
   const long tiles = ((M + BM - 1) / BM) * ((N + BN - 1) / BN);
-  if (split)
-    hipLaunchKernelGGL(gemm_bf16_kernel<6>, dim3((unsigned)tiles), dim3(NT), 0, s, (int)M, (int)N,
-                       (int)Kp, alpha, Ac[0], Ac[1], Ac[2], Kp, Bc[0], Bc[1], Bc[2], Kp, beta, C,
-                       ldc);
-  else
-    hipLaunchKernelGGL(gemm_bf16_kernel<1>, dim3((unsigned)tiles), dim3(NT), 0, s, (int)M, (int)N,
-                       (int)Kp, alpha, Ac[0], Ac[0], Ac[0], Kp, Bc[0], Bc[0], Bc[0], Kp, beta, C,
-                       ldc);
+  const bool vec = reinterpret_cast<uintptr_t>(C) % 16 == 0 && ldc % 4 == 0;
+  auto kern = split ? (vec ? gemm_bf16_kernel<6, true> : gemm_bf16_kernel<6, false>)
+                    : (vec ? gemm_bf16_kernel<1, true> : gemm_bf16_kernel<1, false>);
+  const int c1 = split ? 1 : 0, c2 = split ? 2 : 0;
+  hipLaunchKernelGGL(kern, dim3((unsigned)tiles), dim3(NT), 0, s, (int)M, (int)N, (int)Kp, alpha,
+                     Ac[0], Ac[c1], Ac[c2], Kp, Bc[0], Bc[c1], Bc[c2], Kp, beta, C, ldc);
   ATLG_LAUNCHED();
 }
 

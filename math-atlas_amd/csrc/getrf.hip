@@ -1248,10 +1248,10 @@ void apply_block_lists(LU<S>& f, long k0, long kb, long c0, long n0, long c1, lo
 // Step k then, on the caller's stream:
 //   1. brings block k+1 up to date: block k's interchanges, U = L_kk^-1 A,
 //      and the GEMM update (the critical path);
-//   2. hands block k+1 to the auxiliary stream, which factors it;
-//   3. meanwhile applies block k's interchanges to the columns left of block
-//      k and right of block k+1, and their TRSM and GEMM update (the GEMM on
-//      a capped grid, which leaves compute units to the panel);
+//   2. applies block k's interchanges to the columns left of block k and
+//      right of block k+1, and their TRSM and GEMM update;
+//   3. hands block k+1 to the auxiliary stream, which factors it while 2
+//      runs (it waits for 1 only);
 //   4. waits for the panel before step k+1.
 // The columns each stream touches are disjoint while both run.  Partial
 // pivoting picks from the same updated columns as the plain recursion, so
@@ -1295,12 +1295,9 @@ void getrf_lookahead(LU<S>& f0) {
       node_trsm_cols(f0, k0, kb, k1, nb1);
       node_update(f0, M - k1, nb1, kb, at(f0, k1, k0), at(f0, k0, k1), at(f0, k1, k1));
       ATLG_HIP(hipEventRecord(e_upd, s0));
-      ATLG_HIP(hipStreamWaitEvent(s1, e_upd, 0));
-      f1.win_lo = k1;
-      f1.win_hi = k1 + nb1;
-      getrf_rec(f1, k1, nb1);
-      ATLG_HIP(hipEventRecord(e_panel, s1));
     }
+    // The rest of step k is queued before the panel's many small launches,
+    // so s0 has work while the host issues them.
     const long r0 = k1 + nb1, rest = N - r0;
     apply_block_lists(f0, k0, kb, 0, k0, r0, rest);
     if (rest > 0) {
@@ -1311,7 +1308,14 @@ void getrf_lookahead(LU<S>& f0) {
         f0.gemm_cap = 0;
       }
     }
-    if (nb1 > 0) ATLG_HIP(hipStreamWaitEvent(s0, e_panel, 0));
+    if (nb1 > 0) {
+      ATLG_HIP(hipStreamWaitEvent(s1, e_upd, 0));
+      f1.win_lo = k1;
+      f1.win_hi = k1 + nb1;
+      getrf_rec(f1, k1, nb1);
+      ATLG_HIP(hipEventRecord(e_panel, s1));
+      ATLG_HIP(hipStreamWaitEvent(s0, e_panel, 0));
+    }
   }
   if (split) {  // every panel was joined into s0 above
     ATLG_HIP(hipEventRecord(aux_event(3), s0));
