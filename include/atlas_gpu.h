@@ -101,7 +101,9 @@ int atlg_dlogdet_chol_adjoint(long N, const double *L, long ldl,
  *   the six products above FP32 resolution (six BF16 MFMAs, FP32
  *   accuracy).
  * atlg_sgetrf: as atlg_dgetrf in FP32; gemm_prec 0 = FP32 trailing updates,
- *   1 = BF16 trailing updates with FP32 accumulation, 2 = split BF16 (x6).
+ *   1 = BF16 trailing updates with FP32 accumulation, 2 = split BF16 (x6);
+ *   with 1 and 2 the updates of depth k >= 256 take the BF16 form, the
+ *   narrower ones inside panels stay FP32.
  * atlg_sgetrs: as atlg_dgetrs in FP32.
  * atlg_dsgesv: LAPACK dsgesv semantics.  Factors float(A) with atlg_sgetrf
  *   (gemm_prec = lowprec: 0 FP32, 1 BF16, 2 split BF16), refines X with FP64 residuals B - A X until

@@ -1009,9 +1009,11 @@ inline void node_update(LU<double>& f, long m, long n, long k, const double* A, 
   gemm<double>(f.s, kNoTrans, kNoTrans, m, n, k, -1.0, A, f.lda, B, f.lda, 1.0, C, f.lda, 0,
                nullptr, f.gemm_cap);
 }
+// The BF16 forms (two operand conversions and the product) serve the wide
+// updates, k >= 256; narrower ones inside the panels stay on the FP32 MFMA.
 inline void node_update(LU<float>& f, long m, long n, long k, const float* A, float* B,
                         float* C) {
-  if (f.gemm_prec == kGemmBf16 || f.gemm_prec == kGemmBf16x6)
+  if ((f.gemm_prec == kGemmBf16 || f.gemm_prec == kGemmBf16x6) && k >= 256)
     gemm_bf16(f.s, m, n, k, -1.0f, A, f.lda, B, f.lda, 1.0f, C, f.lda,
               f.gemm_prec == kGemmBf16x6);
   else

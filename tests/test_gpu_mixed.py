@@ -117,6 +117,27 @@ def test_sgemm_bf16(G, m, n, k, split):
         assert err32 > 1e3 * err.max()
 
 
+@pytest.mark.parametrize("split", [0, 1])
+@pytest.mark.parametrize("m,n,k", [(300, 200, 100), (1031, 517, 64)])
+def test_sgemm_bf16_beta0(G, m, n, k, split):
+    # beta = 0 overwrites C without reading it (NaN in C does not propagate);
+    # m = 1031 takes the scalar C path (ldc not a multiple of 4)
+    import torch
+    rng = np.random.default_rng(m * 3 + k)
+    a = rng.uniform(-1, 1, (m, k)).astype(np.float32)
+    b = rng.uniform(-1, 1, (k, n)).astype(np.float32)
+    da, db = dev(a), dev(b)
+    dc = dev(np.full((m, n), np.nan, dtype=np.float32))
+    G.sgemm_bf16(m, n, k, 2.0, da, m, db, k, 0.0, dc, m, split)
+    torch.cuda.synchronize()
+    got = host(dc, m, n, m)
+    ref = 2.0 * (a.astype(np.float64) @ b.astype(np.float64))
+    mag = 2.0 * (np.abs(a).astype(np.float64) @ np.abs(b).astype(np.float64))
+    tol = mag * ((6 * k + 2) * EPS32 if split else (k + 2) * 2.0 ** -8)
+    assert np.all(np.isfinite(got))
+    assert np.all(np.abs(got - ref) <= tol + 1e-30)
+
+
 def lu_ratio(a0, lu, ipiv, eps):
     m, n = a0.shape
     mn = min(m, n)
@@ -131,8 +152,11 @@ def lu_ratio(a0, lu, ipiv, eps):
 
 
 @pytest.mark.parametrize("prec", [0, 1, 2])
-@pytest.mark.parametrize("m,n", [(700, 700), (1536, 1536), (2000, 901), (901, 1300)])
+@pytest.mark.parametrize("m,n", [(700, 700), (1536, 1536), (2000, 901), (901, 1300),
+                                 (2304, 2304), (2600, 2100)])
 def test_sgetrf(G, prec, m, n):
+    # orders >= 2048 take the look-ahead driver, whose 512-deep updates use
+    # the BF16 forms when prec != 0
     import torch
     rng = np.random.default_rng(m + n + prec)
     a = rng.uniform(-1, 1, (m, n)).astype(np.float32)
