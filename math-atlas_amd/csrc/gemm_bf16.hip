@@ -37,6 +37,15 @@ typedef unsigned short us8 __attribute__((ext_vector_type(8)));
 constexpr int BM = 128, BN = 128, NT = 256;
 constexpr int WM = 64, WN = 64, TM = WM / 16, TN = WN / 16;
 constexpr int KPAD = 64;     // K of the converted copies is a multiple of this
+// LDS row padding (BF16 elements, a multiple of 8: rows stay 16-byte aligned)
+// of the single and the split kernel.
+#ifndef ATLG_BF16_PAD1
+#define ATLG_BF16_PAD1 8
+#endif
+#ifndef ATLG_BF16_PAD6
+#define ATLG_BF16_PAD6 8
+#endif
+static_assert(ATLG_BF16_PAD1 % 8 == 0 && ATLG_BF16_PAD6 % 8 == 0, "16-byte LDS rows");
 
 // Register layout of v_mfma_f32_16x16x32_bf16: lane l supplies
 // X1[l&15][8(l>>4) .. 8(l>>4)+7] and X2[8(l>>4) .. +7][l&15]; D[i][j] is in
@@ -107,9 +116,13 @@ to_bf16_kernel(long X, long K, long Kp, const float* __restrict__ S, long lds,
 // C[m][n] = alpha * sum_k At[m][k] * Bt[n][k] + beta * C[m][n]; with
 // NPROD = 6 each operand comes as hi/mid/lo copies (A0/A1/A2, B0/B1/B2).
 // VEC: C is 16-byte aligned and ldc a multiple of 4 (vector C accesses).
-// The C tile is read before the K loop, so its latency overlaps the MFMAs.
+// NPROD = 1: double-buffered LDS, the C tile read before the K loop (its
+// latency overlaps the MFMAs).  NPROD = 6: one LDS buffer (60 KB) and C read
+// at the end, which fit two workgroups per compute unit: one computes while
+// the other waits at its barriers.
 template <int NPROD, bool VEC>
-__global__ void __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(1, 2)))
+__global__ void __launch_bounds__(NT)
+__attribute__((amdgpu_waves_per_eu(NPROD == 1 ? 1 : 2, 2)))
 gemm_bf16_kernel(int M, int N, int Kp, float alpha, const __bf16* __restrict__ A0,
                  const __bf16* __restrict__ A1, const __bf16* __restrict__ A2, long lda,
                  const __bf16* __restrict__ B0, const __bf16* __restrict__ B1,
@@ -117,11 +130,13 @@ gemm_bf16_kernel(int M, int N, int Kp, float alpha, const __bf16* __restrict__ A
                  long ldc) {
   constexpr int NOP = (NPROD == 1) ? 1 : 3;  // operand copies per side
   constexpr int BK = (NPROD == 1) ? 64 : 32;
-  constexpr int LDK = BK + 8;                // LDS row pitch in BF16 (16-byte pad)
+  constexpr int LDK = BK + (NPROD == 1 ? ATLG_BF16_PAD1 : ATLG_BF16_PAD6);  // LDS row pitch
   constexpr int VPR = BK / 8;                // 16-byte vectors per row
   constexpr int NV = BM * VPR / NT;          // vectors per thread per operand copy
-  __shared__ __attribute__((aligned(16))) __bf16 As[2][NOP][BM * LDK];
-  __shared__ __attribute__((aligned(16))) __bf16 Bs[2][NOP][BN * LDK];
+  constexpr int NS = (NPROD == 1) ? 2 : 1;   // LDS stages
+  constexpr bool PREC = NPROD == 1;          // C read before the K loop
+  __shared__ __attribute__((aligned(16))) __bf16 As[NS][NOP][BM * LDK];
+  __shared__ __attribute__((aligned(16))) __bf16 Bs[NS][NOP][BN * LDK];
   const int tilesM = (M + BM - 1) / BM;
   const int tm = blockIdx.x % tilesM, tn = blockIdx.x / tilesM;
   const int m0 = tm * BM, n0 = tn * BN;
@@ -163,7 +178,7 @@ gemm_bf16_kernel(int M, int N, int Kp, float alpha, const __bf16* __restrict__ A
     for (int j = 0; j < TN; j++) {
       acc[i][j] = f4{0, 0, 0, 0};
       cin[i][j] = f4{0, 0, 0, 0};
-      if (beta != 0.0f) {
+      if (PREC && beta != 0.0f) {
         const int gm = m0 + wm0 + i * 16 + cm, gn = n0 + wn0 + j * 16 + cn;
         const float* c = C + (size_t)gm + (size_t)gn * ldc;
         if (VEC && gm + 3 < M && gn < N) {
@@ -178,12 +193,19 @@ gemm_bf16_kernel(int M, int N, int Kp, float alpha, const __bf16* __restrict__ A
 
   const int nk = Kp / BK;
   gload(0);
-  lstore(0);
-  __syncthreads();
+  if (NS == 2) {
+    lstore(0);
+    __syncthreads();
+  }
   const int fx = lane & 15, fk = (lane >> 4) * 8;
   for (int kt = 0; kt < nk; kt++) {
-    const int cur = kt & 1;
+    const int cur = NS == 2 ? (kt & 1) : 0;
     const bool more = kt + 1 < nk;
+    if (NS == 1) {  // the previous step's reads are done before the overwrite
+      if (kt > 0) __syncthreads();
+      lstore(0);
+      __syncthreads();
+    }
     if (more) gload((kt + 1) * BK);
 #pragma unroll
     for (int ks = 0; ks < BK; ks += 32) {
@@ -212,8 +234,10 @@ gemm_bf16_kernel(int M, int N, int Kp, float alpha, const __bf16* __restrict__ A
             acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(af[kPA[p]][i], bfr[kPB[p]][j],
                                                                 acc[i][j], 0, 0, 0);
     }
-    if (more) lstore(cur ^ 1);
-    __syncthreads();
+    if (NS == 2) {
+      if (more) lstore(cur ^ 1);
+      __syncthreads();
+    }
   }
 #pragma unroll
   for (int i = 0; i < TM; i++)
@@ -221,6 +245,15 @@ gemm_bf16_kernel(int M, int N, int Kp, float alpha, const __bf16* __restrict__ A
     for (int j = 0; j < TN; j++) {
       const int gm = m0 + wm0 + i * 16 + cm, gn = n0 + wn0 + j * 16 + cn;
       float* c = C + (size_t)gm + (size_t)gn * ldc;
+      if (!PREC && beta != 0.0f) {
+        if (VEC && gm + 3 < M && gn < N) {
+          cin[i][j] = *reinterpret_cast<const f4*>(c);
+        } else if (gn < N) {
+#pragma unroll
+          for (int r = 0; r < 4; r++)
+            if (gm + r < M) cin[i][j][r] = c[r];
+        }
+      }
       f4 v;
 #pragma unroll
       for (int r = 0; r < 4; r++)
