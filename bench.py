@@ -249,7 +249,7 @@ def bench_ir(torch, G, n):
         # GMRES-IR (classical refinement cannot converge from its 2^-8
         # roundoff at this order)
         best = None
-        for lp, name in ((0, "fp32"), (2, "bf16x6"), (1, "bf16")):
+        for lp, name in ((0, "fp32"), (2, "bf16x6"), (3, "bf16x3"), (1, "bf16")):
             for rf, rname in ((0, "ir"), (1, "gmres")):
                 if lp == 1 and rf == 0:
                     continue

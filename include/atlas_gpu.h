@@ -97,16 +97,18 @@ int atlg_dlogdet_chol_adjoint(long N, const double *L, long ldl,
  * atlg_sgemm: as atlg_dgemm in FP32 (FP32 MFMA).
  * atlg_sgemm_bf16: C = alpha*bf16(A)*bf16(B) + beta*C, NoTrans/NoTrans,
  *   operands rounded to bfloat16 (nearest even), FP32 accumulation and C;
- *   split != 0: each operand as hi + mid + lo BF16 terms (all 24 bits),
+ *   split = 1: each operand as hi + mid + lo BF16 terms (all 24 bits),
  *   the six products above FP32 resolution (six BF16 MFMAs, FP32
- *   accuracy).
+ *   accuracy); split = 2: hi + lo (16 bits), the three products hi*hi,
+ *   hi*lo, lo*hi (about 2^-15 relative error); other values: -12.
  * atlg_sgetrf: as atlg_dgetrf in FP32; gemm_prec 0 = FP32 trailing updates,
- *   1 = BF16 trailing updates with FP32 accumulation, 2 = split BF16 (x6);
- *   with 1 and 2 the updates of depth k >= 256 take the BF16 form, the
- *   narrower ones inside panels stay FP32.
+ *   1 = BF16 trailing updates with FP32 accumulation, 2 = split BF16 (x6),
+ *   3 = two-term split BF16 (x3); with 1-3 the updates of depth k >= 256
+ *   take the BF16 form, the narrower ones inside panels stay FP32.
  * atlg_sgetrs: as atlg_dgetrs in FP32.
  * atlg_dsgesv: LAPACK dsgesv semantics.  Factors float(A) with atlg_sgetrf
- *   (gemm_prec = lowprec: 0 FP32, 1 BF16, 2 split BF16), refines X with FP64 residuals B - A X until
+ *   (gemm_prec = lowprec: 0 FP32, 1 BF16, 2 split BF16, 3 two-term split BF16), refines X with
+ *   FP64 residuals B - A X until
  *   ||r_j||_inf < ||x_j||_inf ||A||_inf eps sqrt(N) for every column j (at
  *   most itermax corrections; itermax <= 0 means 30).  *iter (host) = the
  *   corrections used, or -2 (A overflows FP32), -3 (FP32 factor singular),

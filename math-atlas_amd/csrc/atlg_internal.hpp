@@ -129,8 +129,9 @@ void laswp(hipStream_t s, long N, S* A, long lda, int K1, int K2, const int* ipi
 
 // Precision of getrf's trailing-update GEMMs: the scalar type's own MFMA,
 // or (float only) BF16 operands with FP32 accumulation, single or split
-// (hi + mid + lo, six products: FP32 accuracy) operands.
-enum GemmPrec : int { kGemmNative = 0, kGemmBf16 = 1, kGemmBf16x6 = 2 };
+// (hi + mid + lo, six products: FP32 accuracy; hi + lo, three products:
+// about 2^-15) operands.
+enum GemmPrec : int { kGemmNative = 0, kGemmBf16 = 1, kGemmBf16x6 = 2, kGemmBf16x3 = 3 };
 template <class S>
 void getrf(hipStream_t s, long M, long N, S* A, long lda, int* ipiv, int* dinfo,
            int gemm_prec = kGemmNative);
@@ -178,14 +179,16 @@ void dsposv_batched(hipStream_t s, int nsys, const int* n, double* const* A, con
                     int nrhs, int* info, int* iter, int itermax, const BatchTune& tune);
 
 // C(f32) = alpha * bf16(A) * bf16(B) + beta * C, A M x K and B K x N
-// column-major FP32 (NoTrans, NoTrans), FP32 accumulation.  split: each
-// operand as hi + mid + lo BF16 terms, the six products above FP32
-// resolution (FP32 accuracy).
+// column-major FP32 (NoTrans, NoTrans), FP32 accumulation.  terms = 1: one
+// BF16 term per operand; 3: hi + mid + lo, the six products above FP32
+// resolution (FP32 accuracy); 2: hi + lo, the three products hi*hi, hi*lo,
+// lo*hi (about 2^-15 relative error).
 void gemm_bf16(hipStream_t s, long M, long N, long K, float alpha, const float* A, long lda,
-               const float* B, long ldb, float beta, float* C, long ldc, bool split = false);
+               const float* B, long ldb, float beta, float* C, long ldc, int terms = 1);
 
 // Mixed-precision dgesv (LAPACK dsgesv semantics): FP32 LU (lowprec 1: BF16
-// trailing updates, 2: split BF16), FP64 residual refinement (refine 0:
+// trailing updates, 2: three-term split BF16, 3: two-term split BF16), FP64
+// residual refinement (refine 0:
 // classical, 1: GMRES-IR), FP64 fallback.  *iter_out receives ITER,
 // *inner_out (if not null) the GMRES iterations.  Synchronizes the stream.
 int dsgesv(hipStream_t s, long N, long NRHS, double* A, long lda, int* ipiv, const double* B,

@@ -12,15 +12,17 @@
 // resolution are dropped: "bf16x6" of Henry, Tang and Heinecke, "Leveraging
 // the bfloat16 artificial intelligence datatype for higher-precision
 // computations", ARITH 2019).  Six BF16 MFMAs per tile step give FP32
-// accuracy.
+// accuracy.  Two-term variant (NPROD = 3, "bf16x3" of the same paper):
+// hi = bf16(x), lo = bf16(x - hi) (16 significand bits) and
+// hi*hi + hi*lo + lo*hi, about 2^-15 relative error per product.
 //
 // The operands are first converted into K-contiguous BF16 copies (A
 // transposed through LDS; B's columns are already K-contiguous), with K
 // padded to the tile depth with zeros.  The GEMM then streams 16-byte
 // vectors of 8 BF16 into double-buffered LDS tiles and runs
 // v_mfma_f32_16x16x32_bf16: 128x128 C per 256-thread workgroup, 64x64 per
-// wave, K consumed 64 at a time (32 for the split variant, whose three
-// tiles per operand share the LDS).
+// wave, K consumed 64 at a time (32 for the split variants, whose two or
+// three tiles per operand share the LDS).
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -38,7 +40,7 @@ constexpr int BM = 128, BN = 128, NT = 256;
 constexpr int WM = 64, WN = 64, TM = WM / 16, TN = WN / 16;
 constexpr int KPAD = 64;     // K of the converted copies is a multiple of this
 // LDS row padding (BF16 elements, a multiple of 8: rows stay 16-byte aligned)
-// of the single and the split kernel.
+// of the single and the split kernels.
 #ifndef ATLG_BF16_PAD1
 #define ATLG_BF16_PAD1 8
 #endif
@@ -56,22 +58,26 @@ static_assert(ATLG_BF16_PAD1 % 8 == 0 && ATLG_BF16_PAD6 % 8 == 0, "16-byte LDS r
 // dst[x][k] (k contiguous, pitch ldd, x < X, k < Kp) = bf16(src(x, k)) where
 // src(x, k) = S[x + k*lds] (TRANS: x contiguous in the source) or
 // S[k + x*lds]; zero for k >= K.
-// With Dmid/Dlo != nullptr also writes the remainders: mid = bf16(x - hi),
-// lo = bf16(x - hi - mid) (each difference is exact in FP32).
-__device__ inline void split3(float v, __bf16* hi, __bf16* mid, __bf16* lo) {
+// With nterm > 1 also writes the remainders to D1 (and D2): two terms
+// hi, bf16(x - hi); three terms hi, mid = bf16(x - hi), lo = bf16(x - hi -
+// mid) (each difference is exact in FP32).
+__device__ inline void split_store(float v, int nterm, __bf16* D0, __bf16* D1, __bf16* D2,
+                                   size_t at) {
   const __bf16 h = (__bf16)v;
-  const float r1 = v - (float)h;
-  const __bf16 m = (__bf16)r1;
-  *hi = h;
-  *mid = m;
-  *lo = (__bf16)(r1 - (float)m);
+  D0[at] = h;
+  if (nterm > 1) {
+    const float r1 = v - (float)h;
+    const __bf16 m = (__bf16)r1;
+    D1[at] = m;
+    if (nterm > 2) D2[at] = (__bf16)(r1 - (float)m);
+  }
 }
 
 template <bool TRANS>
 __global__ void __launch_bounds__(256)
 to_bf16_kernel(long X, long K, long Kp, const float* __restrict__ S, long lds,
-               __bf16* __restrict__ D, long ldd, __bf16* __restrict__ Dmid,
-               __bf16* __restrict__ Dlo) {
+               __bf16* __restrict__ D, long ldd, __bf16* __restrict__ D1,
+               __bf16* __restrict__ D2, int nterm) {
   if (TRANS) {
     __shared__ float tile[64][65];
     const long x0 = (long)blockIdx.x * 64, k0 = (long)blockIdx.y * 64;
@@ -83,43 +89,23 @@ to_bf16_kernel(long X, long K, long Kp, const float* __restrict__ S, long lds,
     __syncthreads();
     for (int r = ty; r < 64; r += 4) {
       const long x = x0 + r, k = k0 + tx;
-      if (x < X && k < Kp) {
-        const float v = tile[tx][r];
-        if (Dmid) {
-          __bf16 h, m, l;
-          split3(v, &h, &m, &l);
-          D[x * ldd + k] = h;
-          Dmid[x * ldd + k] = m;
-          Dlo[x * ldd + k] = l;
-        } else {
-          D[x * ldd + k] = (__bf16)v;
-        }
-      }
+      if (x < X && k < Kp) split_store(tile[tx][r], nterm, D, D1, D2, x * ldd + k);
     }
   } else {
     const long x = blockIdx.x;
-    for (long k = threadIdx.x; k < Kp; k += blockDim.x) {
-      const float v = (k < K) ? S[k + x * lds] : 0.0f;
-      if (Dmid) {
-        __bf16 h, m, l;
-        split3(v, &h, &m, &l);
-        D[x * ldd + k] = h;
-        Dmid[x * ldd + k] = m;
-        Dlo[x * ldd + k] = l;
-      } else {
-        D[x * ldd + k] = (__bf16)v;
-      }
-    }
+    for (long k = threadIdx.x; k < Kp; k += blockDim.x)
+      split_store((k < K) ? S[k + x * lds] : 0.0f, nterm, D, D1, D2, x * ldd + k);
   }
 }
 
 // C[m][n] = alpha * sum_k At[m][k] * Bt[n][k] + beta * C[m][n]; with
-// NPROD = 6 each operand comes as hi/mid/lo copies (A0/A1/A2, B0/B1/B2).
+// NPROD = 6 each operand comes as hi/mid/lo copies (A0/A1/A2, B0/B1/B2),
+// with NPROD = 3 as hi/lo copies (A0/A1, B0/B1).
 // VEC: C is 16-byte aligned and ldc a multiple of 4 (vector C accesses).
 // NPROD = 1: double-buffered LDS, the C tile read before the K loop (its
-// latency overlaps the MFMAs).  NPROD = 6: one LDS buffer (60 KB) and C read
-// at the end, which fit two workgroups per compute unit: one computes while
-// the other waits at its barriers.
+// latency overlaps the MFMAs).  NPROD = 3, 6: one LDS buffer (40 / 60 KB)
+// and C read at the end, which fit two workgroups per compute unit: one
+// computes while the other waits at its barriers.
 template <int NPROD, bool VEC>
 __global__ void __launch_bounds__(NT)
 __attribute__((amdgpu_waves_per_eu(NPROD == 1 ? 1 : 2, 2)))
@@ -128,7 +114,8 @@ gemm_bf16_kernel(int M, int N, int Kp, float alpha, const __bf16* __restrict__ A
                  const __bf16* __restrict__ B0, const __bf16* __restrict__ B1,
                  const __bf16* __restrict__ B2, long ldb, float beta, float* __restrict__ C,
                  long ldc) {
-  constexpr int NOP = (NPROD == 1) ? 1 : 3;  // operand copies per side
+  static_assert(NPROD == 1 || NPROD == 3 || NPROD == 6, "product count");
+  constexpr int NOP = NPROD == 1 ? 1 : (NPROD == 3 ? 2 : 3);  // operand copies per side
   constexpr int BK = (NPROD == 1) ? 64 : 32;
   constexpr int LDK = BK + (NPROD == 1 ? ATLG_BF16_PAD1 : ATLG_BF16_PAD6);  // LDS row pitch
   constexpr int VPR = BK / 8;                // 16-byte vectors per row
@@ -221,18 +208,23 @@ gemm_bf16_kernel(int M, int N, int Kp, float alpha, const __bf16* __restrict__ A
           bfr[o][j] =
               *reinterpret_cast<const bf16x8*>(&Bs[cur][o][(wn0 + j * 16 + fx) * LDK + ks + fk]);
       }
-      // Products in the order small terms first: hi*lo, lo*hi, mid*mid,
-      // hi*mid, mid*hi, hi*hi (only hi*hi without the split).  The product
-      // loop is outermost so consecutive MFMAs go to different accumulators.
-      constexpr int kPA[6] = {0, 2, 1, 0, 1, 0}, kPB[6] = {2, 0, 1, 1, 0, 0};
+      // Products in the order small terms first: (six) hi*lo, lo*hi,
+      // mid*mid, hi*mid, mid*hi, hi*hi; (three) hi*lo, lo*hi, hi*hi; (one)
+      // hi*hi.  The product loop is outermost so consecutive MFMAs go to
+      // different accumulators.
+      constexpr int kPA6[6] = {0, 2, 1, 0, 1, 0}, kPB6[6] = {2, 0, 1, 1, 0, 0};
+      constexpr int kPA3[3] = {0, 1, 0}, kPB3[3] = {1, 0, 0};
 #pragma unroll
-      for (int p = 6 - NPROD; p < 6; p++)
+      for (int p = 0; p < NPROD; p++) {
+        const int pa = NPROD == 6 ? kPA6[p] : (NPROD == 3 ? kPA3[p] : 0);
+        const int pb = NPROD == 6 ? kPB6[p] : (NPROD == 3 ? kPB3[p] : 0);
 #pragma unroll
         for (int i = 0; i < TM; i++)
 #pragma unroll
           for (int j = 0; j < TN; j++)
-            acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(af[kPA[p]][i], bfr[kPB[p]][j],
-                                                                acc[i][j], 0, 0, 0);
+            acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(af[pa][i], bfr[pb][j], acc[i][j],
+                                                                0, 0, 0);
+      }
     }
     if (NS == 2) {
       if (more) lstore(cur ^ 1);
@@ -271,7 +263,7 @@ gemm_bf16_kernel(int M, int N, int Kp, float alpha, const __bf16* __restrict__ A
 }  // namespace
 
 void gemm_bf16(hipStream_t s, long M, long N, long K, float alpha, const float* A, long lda,
-               const float* B, long ldb, float beta, float* C, long ldc, bool split) {
+               const float* B, long ldb, float beta, float* C, long ldc, int terms) {
   if (M <= 0 || N <= 0) return;
   if (K <= 0 || alpha == 0.0f) {
     scale<float>(s, M, N, beta, C, ldc);
@@ -280,7 +272,7 @@ void gemm_bf16(hipStream_t s, long M, long N, long K, float alpha, const float* 
   const long Kp = (K + KPAD - 1) / KPAD * KPAD;
   const size_t ba = ((size_t)M * Kp * sizeof(__bf16) + 255) & ~size_t(255);
   const size_t bb = ((size_t)N * Kp * sizeof(__bf16) + 255) & ~size_t(255);
-  const int copies = split ? 3 : 1;
+  const int copies = terms;  // BF16 copies per operand: 1, 2 (hi, lo) or 3 (hi, mid, lo)
   char* w = static_cast<char*>(scratch(s, kSlotBf16, copies * (ba + bb)));
   __bf16* Ac[3];
   __bf16* Bc[3];
@@ -289,19 +281,18 @@ void gemm_bf16(hipStream_t s, long M, long N, long K, float alpha, const float* 
     Bc[c] = reinterpret_cast<__bf16*>(w + (size_t)(c < copies ? c : 0) * (ba + bb) + ba);
   }
   hipLaunchKernelGGL(to_bf16_kernel<true>, dim3((unsigned)((M + 63) / 64), (unsigned)(Kp / 64)),
-                     dim3(256), 0, s, M, K, Kp, A, lda, Ac[0], Kp, split ? Ac[1] : nullptr,
-                     split ? Ac[2] : nullptr);
+                     dim3(256), 0, s, M, K, Kp, A, lda, Ac[0], Kp, Ac[1], Ac[2], terms);
   ATLG_LAUNCHED();
   hipLaunchKernelGGL(to_bf16_kernel<false>, dim3((unsigned)N), dim3(256), 0, s, N, K, Kp, B, ldb,
-                     Bc[0], Kp, split ? Bc[1] : nullptr, split ? Bc[2] : nullptr);
+                     Bc[0], Kp, Bc[1], Bc[2], terms);
   ATLG_LAUNCHED();
   const long tiles = ((M + BM - 1) / BM) * ((N + BN - 1) / BN);
   const bool vec = reinterpret_cast<uintptr_t>(C) % 16 == 0 && ldc % 4 == 0;
-  auto kern = split ? (vec ? gemm_bf16_kernel<6, true> : gemm_bf16_kernel<6, false>)
-                    : (vec ? gemm_bf16_kernel<1, true> : gemm_bf16_kernel<1, false>);
-  const int c1 = split ? 1 : 0, c2 = split ? 2 : 0;
+  auto kern = terms == 3   ? (vec ? gemm_bf16_kernel<6, true> : gemm_bf16_kernel<6, false>)
+              : terms == 2 ? (vec ? gemm_bf16_kernel<3, true> : gemm_bf16_kernel<3, false>)
+                           : (vec ? gemm_bf16_kernel<1, true> : gemm_bf16_kernel<1, false>);
   hipLaunchKernelGGL(kern, dim3((unsigned)tiles), dim3(NT), 0, s, (int)M, (int)N, (int)Kp, alpha,
-                     Ac[0], Ac[c1], Ac[c2], Kp, Bc[0], Bc[c1], Bc[c2], Kp, beta, C, ldc);
+                     Ac[0], Ac[1], Ac[2], Kp, Bc[0], Bc[1], Bc[2], Kp, beta, C, ldc);
   ATLG_LAUNCHED();
 }
 

@@ -1013,9 +1013,9 @@ inline void node_update(LU<double>& f, long m, long n, long k, const double* A, 
 // updates, k >= 256; narrower ones inside the panels stay on the FP32 MFMA.
 inline void node_update(LU<float>& f, long m, long n, long k, const float* A, float* B,
                         float* C) {
-  if ((f.gemm_prec == kGemmBf16 || f.gemm_prec == kGemmBf16x6) && k >= 256)
+  if (f.gemm_prec != kGemmNative && k >= 256)
     gemm_bf16(f.s, m, n, k, -1.0f, A, f.lda, B, f.lda, 1.0f, C, f.lda,
-              f.gemm_prec == kGemmBf16x6);
+              f.gemm_prec == kGemmBf16x6 ? 3 : (f.gemm_prec == kGemmBf16x3 ? 2 : 1));
   else
     gemm<float>(f.s, kNoTrans, kNoTrans, m, n, k, -1.0f, A, f.lda, B, f.lda, 1.0f, C, f.lda, 0,
                 nullptr, f.gemm_cap);

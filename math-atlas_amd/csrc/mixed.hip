@@ -417,7 +417,10 @@ int dsgesv(hipStream_t s, long N, long NRHS, double* A, long lda, int* ipiv, con
                      small + 2 * nr);
   ATLG_LAUNCHED();
   getrf<float>(s, N, N, A32, N, ipiv, flags + 1,
-               lowprec == 1 ? kGemmBf16 : (lowprec == 2 ? kGemmBf16x6 : kGemmNative));
+               lowprec == 1   ? kGemmBf16
+               : lowprec == 2 ? kGemmBf16x6
+               : lowprec == 3 ? kGemmBf16x3
+                              : kGemmNative);
   std::vector<int> hpiv(N);
   ATLG_HIP(hipMemcpyAsync(hs.data(), small, sizeof(unsigned long long) * nsmall,
                           hipMemcpyDeviceToHost, s));

@@ -326,8 +326,10 @@ int atlg_sgemm_bf16(long M, long N, long K, float alpha, const float* A, long ld
   if (lda < std::max(1L, M)) return -6;
   if (ldb < std::max(1L, K)) return -8;
   if (ldc < std::max(1L, M)) return -11;
+  if (split < 0 || split > 2) return -12;
   return guarded([&] {
-    gemm_bf16(resolve_stream(stream), M, N, K, alpha, A, lda, B, ldb, beta, C, ldc, split != 0);
+    gemm_bf16(resolve_stream(stream), M, N, K, alpha, A, lda, B, ldb, beta, C, ldc,
+              split == 1 ? 3 : (split == 2 ? 2 : 1));
   });
 }
 
@@ -337,7 +339,7 @@ int atlg_sgetrf(long M, long N, float* A, long lda, int* ipiv, int* dinfo, int g
   if (N < 0) return -2;
   if (lda < std::max(1L, M)) return -4;
   if (!dinfo) return -6;
-  if (gemm_prec != kGemmNative && gemm_prec != kGemmBf16 && gemm_prec != kGemmBf16x6) return -7;
+  if (gemm_prec < kGemmNative || gemm_prec > kGemmBf16x3) return -7;
   return guarded(
       [&] { getrf<float>(resolve_stream(stream), M, N, A, lda, ipiv, dinfo, gemm_prec); });
 }
@@ -362,7 +364,7 @@ int atlg_dsgesv(long N, long NRHS, double* A, long lda, int* ipiv, const double*
   if (ldb < std::max(1L, N)) return -7;
   if (ldx < std::max(1L, N)) return -9;
   if (!dinfo) return -10;
-  if (lowprec < 0 || lowprec > 2) return -12;
+  if (lowprec < 0 || lowprec > 3) return -12;
   if (refine != 0 && refine != 1) return -13;
   if (!iter) return -14;
   return guarded([&] {

@@ -96,6 +96,23 @@ def test_device_api_rejects_bad_arguments_without_gpu(G):
     assert L.atlg_dpotrf(G.Lower, 4, nul, 2, nul, nul) == -4
 
 
+def test_mixed_precision_argument_errors(G):
+    """Precision selectors outside their ranges are rejected on the host."""
+    L = G.lib()
+    c = ctypes
+    s = c.c_void_p(0)
+    p = c.c_void_p(16)
+    it, inn = c.c_int(0), c.c_int(0)
+    assert L.atlg_sgemm_bf16(4, 4, 4, 1.0, p, 4, p, 4, 0.0, p, 4, 3, s) == -12
+    assert L.atlg_sgemm_bf16(4, 4, 4, 1.0, p, 4, p, 4, 0.0, p, 4, -1, s) == -12
+    assert L.atlg_sgetrf(4, 4, p, 4, p, p, 4, s) == -7
+    assert L.atlg_sgetrf(4, 4, p, 4, p, p, -1, s) == -7
+    assert L.atlg_dsgesv(4, 1, p, 4, p, p, 4, p, 4, p, 30, 4, 0, c.byref(it), c.byref(inn),
+                         s) == -12
+    assert L.atlg_dsgesv(4, 1, p, 4, p, p, 4, p, 4, p, 30, 3, 2, c.byref(it), c.byref(inn),
+                         s) == -13
+
+
 def test_batched_argument_errors(G):
     """The batched entry points check every system's order and leading
     dimensions on the host before any device work."""

@@ -26,6 +26,7 @@ pytestmark = pytest.mark.gpu
 
 EPS32 = np.finfo(np.float32).eps / 2
 U_BF16 = 2.0 ** -8
+U_BF16X3 = 2.0 ** -15  # two-term split operands: error of one product
 
 
 @pytest.fixture(scope="module")
@@ -74,7 +75,7 @@ def test_sgemm(G, ta, tb, m, n, k):
     assert np.all(np.abs(got - ref) <= bound + 1e-30)
 
 
-@pytest.mark.parametrize("split", [0, 1])
+@pytest.mark.parametrize("split", [0, 1, 2])
 @pytest.mark.parametrize("m,n,k", [(300, 200, 100), (128, 256, 64), (1031, 517, 777)])
 def test_sgemm_bf16(G, m, n, k, split):
     import torch
@@ -88,13 +89,19 @@ def test_sgemm_bf16(G, m, n, k, split):
     got = host(dc, m, n, m)
     a_hi, b_hi = to_bf16(a), to_bf16(b)
     a16, b16 = a_hi.astype(np.float64), b_hi.astype(np.float64)
-    if split:
+    if split == 1:
         a_mid = to_bf16(a - a_hi)
         b_mid = to_bf16(b - b_hi)
         a_lo = to_bf16(a - a_hi - a_mid).astype(np.float64)
         b_lo = to_bf16(b - b_hi - b_mid).astype(np.float64)
         a_mid, b_mid = a_mid.astype(np.float64), b_mid.astype(np.float64)
         terms = [(a16, b16), (a16, b_mid), (a_mid, b16), (a_mid, b_mid), (a16, b_lo), (a_lo, b16)]
+        prod = sum(x @ y for x, y in terms)
+        mag = sum(np.abs(x) @ np.abs(y) for x, y in terms)
+    elif split == 2:
+        a_lo = to_bf16(a - a_hi).astype(np.float64)
+        b_lo = to_bf16(b - b_hi).astype(np.float64)
+        terms = [(a16, b16), (a16, b_lo), (a_lo, b16)]
         prod = sum(x @ y for x, y in terms)
         mag = sum(np.abs(x) @ np.abs(y) for x, y in terms)
     else:
@@ -108,16 +115,22 @@ def test_sgemm_bf16(G, m, n, k, split):
     assert np.all(err <= bound + 1e-30)
     ref32 = c.astype(np.float64) - a.astype(np.float64) @ b.astype(np.float64)
     err32 = np.abs(got - ref32).max()
-    if split:
+    mag32 = np.abs(a.astype(np.float64)) @ np.abs(b.astype(np.float64)) + np.abs(c)
+    if split == 1:
         # FP32 accuracy: within the FP32 GEMM element bound of the true product
-        b32 = (np.abs(a.astype(np.float64)) @ np.abs(b.astype(np.float64)) + np.abs(c)) * (k + 2) * EPS32
-        assert np.all(np.abs(got - ref32) <= 2 * b32)
+        assert np.all(np.abs(got - ref32) <= 2 * mag32 * (k + 2) * EPS32)
+    elif split == 2:
+        # 16 significand bits per operand: about 2^-15 per product, far
+        # below single BF16 and above FP32
+        assert np.all(np.abs(got - ref32) <= mag32 * (2.0 ** -14 + (k + 2) * EPS32))
+        # the lo terms were included: far closer than one BF16 term per operand
+        assert err32 < 0.1 * np.abs(a16 @ b16 - a.astype(np.float64) @ b.astype(np.float64)).max()
     else:
         # the BF16 rounding really happened: an FP32 product differs
         assert err32 > 1e3 * err.max()
 
 
-@pytest.mark.parametrize("split", [0, 1])
+@pytest.mark.parametrize("split", [0, 1, 2])
 @pytest.mark.parametrize("m,n,k", [(300, 200, 100), (1031, 517, 64)])
 def test_sgemm_bf16_beta0(G, m, n, k, split):
     # beta = 0 overwrites C without reading it (NaN in C does not propagate);
@@ -133,7 +146,8 @@ def test_sgemm_bf16_beta0(G, m, n, k, split):
     got = host(dc, m, n, m)
     ref = 2.0 * (a.astype(np.float64) @ b.astype(np.float64))
     mag = 2.0 * (np.abs(a).astype(np.float64) @ np.abs(b).astype(np.float64))
-    tol = mag * ((6 * k + 2) * EPS32 if split else (k + 2) * 2.0 ** -8)
+    tol = mag * ({0: (k + 2) * 2.0 ** -8, 1: (6 * k + 2) * EPS32,
+                  2: 2.0 ** -14 + (6 * k + 2) * EPS32}[split])
     assert np.all(np.isfinite(got))
     assert np.all(np.abs(got - ref) <= tol + 1e-30)
 
@@ -151,7 +165,7 @@ def lu_ratio(a0, lu, ipiv, eps):
     return np.abs(pa - L @ U).sum(axis=0).max() / (np.abs(a0).sum(axis=0).max() * eps * mn)
 
 
-@pytest.mark.parametrize("prec", [0, 1, 2])
+@pytest.mark.parametrize("prec", [0, 1, 2, 3])
 @pytest.mark.parametrize("m,n", [(700, 700), (1536, 1536), (2000, 901), (901, 1300),
                                  (2304, 2304), (2600, 2100)])
 def test_sgetrf(G, prec, m, n):
@@ -167,7 +181,8 @@ def test_sgetrf(G, prec, m, n):
     torch.cuda.synchronize()
     assert int(info.item()) == 0
     lu = host(da, m, n, m)
-    r = lu_ratio(a.astype(np.float64), lu, ipiv.cpu().numpy(), U_BF16 if prec == 1 else EPS32)
+    r = lu_ratio(a.astype(np.float64), lu, ipiv.cpu().numpy(),
+                 {0: EPS32, 1: U_BF16, 2: EPS32, 3: U_BF16X3}[prec])
     print("sgetrf prec=%d %dx%d: ratio %.3g" % (prec, m, n, r))
     assert r <= 100.0
 
@@ -228,7 +243,7 @@ def backward_error(a, x, b):
                  + np.abs(b).max(axis=0))).max()
 
 
-@pytest.mark.parametrize("lowprec", [0, 1, 2])
+@pytest.mark.parametrize("lowprec", [0, 1, 2, 3])
 @pytest.mark.parametrize("n,nrhs,kappa", [(1024, 1, 1e2), (1536, 1, 1e6), (777, 3, 1e2),
                                           (1024, 1, 1e10), (600, 6, 1e6)])
 def test_dsgesv(G, lowprec, n, nrhs, kappa):
@@ -257,11 +272,13 @@ def test_dsgesv(G, lowprec, n, nrhs, kappa):
         assert lu_ratio(a, lu, ipiv.cpu().numpy(), np.finfo(np.float64).eps / 2) <= 100.0
     if kappa <= 1e6 and lowprec in (0, 2):
         assert it >= 0  # FP32-accurate factor: refinement converges for kappa * u32 < 1
-    if kappa >= 1e10 and lowprec in (0, 2):
+    if kappa <= 1e2 and lowprec == 3:
+        assert it >= 0  # two-term split factor: kappa * 2^-15 well below 1
+    if kappa >= 1e10 and lowprec in (0, 2, 3):
         assert it < 0
 
 
-@pytest.mark.parametrize("lowprec", [0, 1, 2])
+@pytest.mark.parametrize("lowprec", [0, 1, 2, 3])
 @pytest.mark.parametrize("n,nrhs,kappa", [(1024, 1, 1e2), (1536, 1, 1e6), (600, 2, 1e6),
                                           (1024, 1, 1e10)])
 def test_dsgesv_gmres_ir(G, lowprec, n, nrhs, kappa):
@@ -291,6 +308,6 @@ def test_dsgesv_gmres_ir(G, lowprec, n, nrhs, kappa):
     # GMRES-IR converges while kappa * u_factor stays well below 1/u_factor:
     # FP32-accurate factors (FP32, split BF16) to kappa = 1e6; the BF16
     # factor (u = 2^-8, beyond the reach of classical refinement already at
-    # kappa = 1e2 for larger n) at kappa = 1e2
+    # kappa = 1e2 for larger n) and the two-term split one at kappa = 1e2
     if kappa <= (1e6 if lowprec in (0, 2) else 1e2):
         assert it >= 0
