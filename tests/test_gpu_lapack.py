@@ -77,6 +77,23 @@ def test_getrf_large_ragged(G):
     assert O.lu_resid(n, n, a0, lu, ipiv) <= 100
 
 
+def test_getrf_repeatable(G):
+    # the look-ahead driver runs panels and updates on two streams over
+    # disjoint columns, and no reduction depends on timing: repeated
+    # factorizations of one matrix agree bit for bit
+    n = 2600
+    a0 = O.gegen(n, n, 4242)
+    outs = []
+    for _ in range(3):
+        lu = a0.copy(order="F")
+        ipiv = np.zeros(n, dtype=np.int32)
+        assert G.clapack_dgetrf(G.ColMajor, n, n, lu, n, ipiv) == 0
+        outs.append((lu, ipiv))
+    for lu, ipiv in outs[1:]:
+        assert np.array_equal(lu, outs[0][0])
+        assert np.array_equal(ipiv, outs[0][1])
+
+
 def test_getrf_known_pivots(G):
     # A = P^T L U with |L_ij| < 0.45/n and diag(U) in [1, 2]: at every step the
     # unit-diagonal row beats every competitor by a factor > n (and L, U are
