@@ -3,6 +3,7 @@
 #include <hip/hip_runtime.h>
 #include <cstddef>
 #include <cstdint>
+#include <mutex>
 #include <string>
 
 #include "atlas_gpu.h"
@@ -61,6 +62,11 @@ hipEvent_t aux_event(int i);  // i = 0..3
 // once per device, on the first call's `cus`; false when cus <= 0 or the
 // masked streams are unavailable.
 bool split_streams(int cus, hipStream_t* small, hipStream_t* large);
+// Held by a look-ahead driver while it queues work on the shared auxiliary
+// streams and events above, so that calls from several host threads do not
+// interleave their event records and waits (the device work of successive
+// calls then runs in queue order on those streams).
+std::unique_lock<std::mutex> aux_lock();
 
 // Per-stream scratch memory in named slots.  A slot grows (after a stream
 // synchronize) and is reused by later calls on the same stream; pointers from

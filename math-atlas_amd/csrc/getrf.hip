@@ -1260,6 +1260,7 @@ void apply_block_lists(LU<S>& f, long k0, long kb, long c0, long n0, long c1, lo
 // the pivots agree wherever they are well separated.
 template <class S>
 void getrf_lookahead(LU<S>& f0) {
+  const auto lock = aux_lock();
   hipStream_t s0 = f0.s, s1 = aux_stream();
   hipEvent_t e_upd = aux_event(0), e_panel = aux_event(1);
   // With ATLG_LA_PANEL_CUS > 0 the panels run on a stream confined to that

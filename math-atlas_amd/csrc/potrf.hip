@@ -249,6 +249,7 @@ constexpr long kCholLaMin = 2048;
 // A22 -= L21 L21^T; upper: A = U^T U, U12 = U11^-T A12, A22 -= U12^T U12.
 void potrf_lookahead(hipStream_t s0, bool lower, long N, double* A, long lda, int* dinfo,
                      double* W) {
+  const auto lock = aux_lock();
   hipStream_t s1 = aux_stream();
   hipEvent_t e_upd = aux_event(0), e_panel = aux_event(1);
   auto at = [&](long i, long j) { return A + i + j * lda; };

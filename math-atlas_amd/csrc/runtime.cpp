@@ -69,6 +69,9 @@ Aux& aux_for_device() {
 }
 }  // namespace
 
+std::mutex g_aux_mu;
+std::unique_lock<std::mutex> aux_lock() { return std::unique_lock<std::mutex>(g_aux_mu); }
+
 hipStream_t aux_stream() { return aux_for_device().stream; }
 hipEvent_t aux_event(int i) { return aux_for_device().ev[i & 3]; }
 

@@ -94,6 +94,51 @@ def test_getrf_repeatable(G):
         assert np.array_equal(ipiv, outs[0][1])
 
 
+def test_getrf_concurrent_host_threads(G):
+    # two host threads factor different matrices on their own streams at
+    # the same time (the look-ahead drivers share auxiliary streams and
+    # events); each result equals the one computed alone
+    import threading
+    import torch
+    n = 2304
+    mats = [torch.from_numpy(O.gegen(n, n, 900 + t).T.copy()).cuda() for t in range(2)]
+    alone = []
+    for a0 in mats:
+        w = a0.clone()
+        ipiv = torch.zeros(n, dtype=torch.int32, device="cuda")
+        info = torch.zeros(1, dtype=torch.int32, device="cuda")
+        G.dgetrf(n, n, w, n, ipiv, info)
+        torch.cuda.synchronize()
+        alone.append((w, ipiv))
+    for _ in range(3):
+        outs = [None, None]
+        errs = []
+
+        def work(t):
+            try:
+                st = torch.cuda.Stream()
+                with torch.cuda.stream(st):
+                    w = mats[t].clone()
+                    ipiv = torch.zeros(n, dtype=torch.int32, device="cuda")
+                    info = torch.zeros(1, dtype=torch.int32, device="cuda")
+                    G.dgetrf(n, n, w, n, ipiv, info, stream=st)
+                st.synchronize()
+                outs[t] = (w, ipiv, int(info.item()))
+            except Exception as e:  # reported below, on the main thread
+                errs.append(e)
+
+        th = [threading.Thread(target=work, args=(t,)) for t in range(2)]
+        for x in th:
+            x.start()
+        for x in th:
+            x.join()
+        assert not errs, errs
+        for t in range(2):
+            assert outs[t][2] == 0
+            assert torch.equal(outs[t][0], alone[t][0])
+            assert torch.equal(outs[t][1], alone[t][1])
+
+
 def test_getrf_known_pivots(G):
     # A = P^T L U with |L_ij| < 0.45/n and diag(U) in [1, 2]: at every step the
     # unit-diagonal row beats every competitor by a factor > n (and L, U are
