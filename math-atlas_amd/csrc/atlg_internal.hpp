@@ -59,8 +59,8 @@ hipStream_t aux_stream();
 hipEvent_t aux_event(int i);  // i = 0..3
 // A pair of streams restricted to disjoint compute units: *small gets the
 // first `cus` units, *large the rest (hipExtStreamCreateWithCUMask).  Made
-// once per device, on the first call's `cus`; false when cus <= 0 or the
-// masked streams are unavailable.
+// once per device and unit count; false when cus <= 0 or the masked streams
+// are unavailable.
 bool split_streams(int cus, hipStream_t* small, hipStream_t* large);
 // Held by a look-ahead driver while it queues work on the shared auxiliary
 // streams and events above, so that calls from several host threads do not

@@ -229,16 +229,23 @@ void potrf_rec(hipStream_t s, bool lower, long N, double* A, long lda, int* dinf
   potrf_rec(s, lower, nr, A22, lda, dinfo, off + (int)nl, W);
 }
 
-// Look-ahead (1) or plain right-looking steps (0).  Measured at N = 16384
-// both take the same time: the diagonal-block chain (two 128-block leaf
-// kernels and the panel solve per step) is the longer of the two paths, so
-// overlapping the trailing SYRK with it gains nothing.
-// Grid cap of the look-ahead's trailing SYRK (workgroups).
+// Look-ahead (1) or plain right-looking steps (0).  At N = 16384 the
+// look-ahead with the panels on 32 compute units and the SYRKs on the rest
+// takes 55 ms against 65 ms for plain steps; on a shared device (no masked
+// streams) both took 65 ms: the 128-block leaves, which need 132 KB of LDS,
+// wait for a compute unit free of SYRK workgroups.
+// Without masked streams: grid cap of the look-ahead's trailing SYRK
+// (workgroups).
 #ifndef ATLG_CHOL_LOOKAHEAD
-#define ATLG_CHOL_LOOKAHEAD 0
+#define ATLG_CHOL_LOOKAHEAD 1
 #endif
 #ifndef ATLG_CHOL_CAP
 #define ATLG_CHOL_CAP 384
+#endif
+// Compute units of the panel stream when the look-ahead runs panels and
+// updates on a CU-masked stream pair (0: both share the device).
+#ifndef ATLG_CHOL_PANEL_CUS
+#define ATLG_CHOL_PANEL_CUS 32
 #endif
 constexpr long kCholLaMin = 2048;
 
@@ -252,6 +259,17 @@ void potrf_lookahead(hipStream_t s0, bool lower, long N, double* A, long lda, in
   const auto lock = aux_lock();
   hipStream_t s1 = aux_stream();
   hipEvent_t e_upd = aux_event(0), e_panel = aux_event(1);
+  // with a CU-masked pair the 128-block leaves (132 KB of LDS) never wait
+  // for a compute unit free of SYRK workgroups
+  hipStream_t caller = s0, sp = nullptr, sg = nullptr;
+  const bool split = ATLG_CHOL_PANEL_CUS > 0 && split_streams(ATLG_CHOL_PANEL_CUS, &sp, &sg);
+  if (split) {
+    ATLG_HIP(hipEventRecord(aux_event(2), caller));
+    ATLG_HIP(hipStreamWaitEvent(sg, aux_event(2), 0));
+    ATLG_HIP(hipStreamWaitEvent(sp, aux_event(2), 0));
+    s0 = sg;
+    s1 = sp;
+  }
   auto at = [&](long i, long j) { return A + i + j * lda; };
   // the panel of the block at k (kb columns) below/right of it, rows/cols r
   auto panel_ptr = [&](long k, long kb) { return lower ? at(k + kb, k) : at(k, k + kb); };
@@ -285,16 +303,20 @@ void potrf_lookahead(hipStream_t s0, bool lower, long N, double* A, long lda, in
   for (long k0 = 0; k0 + kb0 < N; k0 += RB) {
     const long kb = std::min(RB, N - k0), k1 = k0 + kb, nb1 = std::min(RB, N - k1);
     const long rest = N - k1 - nb1;
-    // block k+1 first (all its rows), then hand it to the auxiliary stream
+    // block k+1 first (all its rows), then the rest of step k's update,
+    // beside which the auxiliary stream factors block k+1 and its panel
     update(s0, k0, kb, k1, N - k1, nb1, 0);
     ATLG_HIP(hipEventRecord(e_upd, s0));
+    update(s0, k0, kb, k1 + nb1, rest, rest, split ? 0 : ATLG_CHOL_CAP);
     ATLG_HIP(hipStreamWaitEvent(s1, e_upd, 0));
     potrf_rec(s1, lower, nb1, at(k1, k1), lda, dinfo, (int)k1, W);
     solve_panel(s1, k1, nb1, rest);
     ATLG_HIP(hipEventRecord(e_panel, s1));
-    // the rest of step k's update beside it
-    update(s0, k0, kb, k1 + nb1, rest, rest, ATLG_CHOL_CAP);
     ATLG_HIP(hipStreamWaitEvent(s0, e_panel, 0));
+  }
+  if (split) {  // every panel was joined into s0 above
+    ATLG_HIP(hipEventRecord(aux_event(3), s0));
+    ATLG_HIP(hipStreamWaitEvent(caller, aux_event(3), 0));
   }
 }
 

@@ -52,8 +52,9 @@ namespace {
 struct Aux {
   hipStream_t stream = nullptr;
   hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
-  hipStream_t split[2] = {nullptr, nullptr};  // CU-masked pair, created on first use
-  int split_cus = -1;
+  // CU-masked pairs by the small side's unit count, created on first use
+  // ({nullptr, nullptr}: unavailable)
+  std::map<int, std::pair<hipStream_t, hipStream_t>> split;
 };
 std::map<int, Aux> g_aux;
 Aux& aux_for_device() {
@@ -78,29 +79,29 @@ hipEvent_t aux_event(int i) { return aux_for_device().ev[i & 3]; }
 bool split_streams(int cus, hipStream_t* small, hipStream_t* large) {
   Aux& a = aux_for_device();
   std::lock_guard<std::mutex> lk(g_mu);
-  if (a.split_cus < 0) {
+  auto it = a.split.find(cus);
+  if (it == a.split.end()) {
     int dev = 0, total = 0;
     ATLG_HIP(hipGetDevice(&dev));
     ATLG_HIP(hipDeviceGetAttribute(&total, hipDeviceAttributeMultiprocessorCount, dev));
-    a.split_cus = 0;
+    hipStream_t p[2] = {nullptr, nullptr};
     if (cus > 0 && cus < total) {
       const int words = (total + 31) / 32;
       std::vector<uint32_t> m0(words, 0u), m1(words, 0u);
       for (int c = 0; c < total; c++) (c < cus ? m0 : m1)[c / 32] |= 1u << (c % 32);
-      if (hipExtStreamCreateWithCUMask(&a.split[0], words, m0.data()) == hipSuccess &&
-          hipExtStreamCreateWithCUMask(&a.split[1], words, m1.data()) == hipSuccess) {
-        a.split_cus = cus;
-      } else {
+      if (hipExtStreamCreateWithCUMask(&p[0], words, m0.data()) != hipSuccess ||
+          hipExtStreamCreateWithCUMask(&p[1], words, m1.data()) != hipSuccess) {
         (void)hipGetLastError();
-        for (auto& st : a.split)
+        for (auto& st : p)
           if (st) (void)hipStreamDestroy(st);
-        a.split[0] = a.split[1] = nullptr;
+        p[0] = p[1] = nullptr;
       }
     }
+    it = a.split.emplace(cus, std::make_pair(p[0], p[1])).first;
   }
-  if (a.split_cus <= 0) return false;
-  *small = a.split[0];
-  *large = a.split[1];
+  if (!it->second.first) return false;
+  *small = it->second.first;
+  *large = it->second.second;
   return true;
 }
 

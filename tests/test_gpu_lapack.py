@@ -228,6 +228,27 @@ def test_potrf_info_indefinite(G):
         assert G.clapack_dpotrf(G.ColMajor, G.Lower, n, f, n) == k + 1
 
 
+@pytest.mark.parametrize("uplo", [O.Lower, O.Upper])
+def test_potrf_large_ragged(G, uplo):
+    # orders >= 2048 (block-column steps of 256, ragged last block): llttst,
+    # the unused triangle untouched, and info from a minor in a later block
+    n = 2300
+    lda = n + 3
+    s = O.lltgen(uplo, n, n * 1029 + lda, lda=lda)
+    mask = np.triu(np.ones((n, n), bool), 1) if uplo == O.Lower else np.tril(np.ones((n, n), bool), -1)
+    s[:n][mask] = -973200000.0
+    f = s.copy(order="F")
+    assert G.clapack_dpotrf(G.ColMajor, uplo, n, f, lda) == 0
+    assert np.all(f[:n][mask] == -973200000.0)
+    assert O.gechkgap(f, n) == 0
+    r = O.lib().orc_llt_resid(uplo, n, O._p(s), lda, O._p(f), lda)
+    assert r <= 16.0, (n, r)
+    s2 = O.lltgen(uplo, n, 777)
+    s2[1700, 1700] = -1.0e6
+    f2 = s2.copy(order="F")
+    assert G.clapack_dpotrf(G.ColMajor, uplo, n, f2, n) == 1701
+
+
 def slv_inputs(n, nrhs, lda, ldb):
     a = O.gegen(n, n, n * n + lda, lda=lda)
     x = O.gegen(n, nrhs, n * nrhs + ldb, lda=ldb)
