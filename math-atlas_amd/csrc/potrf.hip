@@ -38,7 +38,7 @@ __device__ __forceinline__ double lane_value(double v, int src) {
 
 constexpr int SB = 32;  // sub-block order of the in-workgroup factorization
 #ifndef ATLG_POTRF_RB
-#define ATLG_POTRF_RB 256
+#define ATLG_POTRF_RB 384
 #endif
 constexpr long RB = ATLG_POTRF_RB;  // block width of the right-looking outer steps
 
