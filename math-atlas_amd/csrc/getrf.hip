@@ -103,10 +103,19 @@ constexpr long kSpinLimit = 1L << 20;  // mailbox polls (each one round trip) be
 #ifndef ATLG_LA_CAP_AREA
 #define ATLG_LA_CAP_AREA (16384L * 16384L)
 #endif
+// Look-ahead block width: ATLG_LA_NB when set (a multiple of 128), else
+// about min(M,N)/32 rounded down to 128, within [512, 1024]: the panel
+// chain's cost per column does not depend on the width, while wider blocks
+// make deeper (more efficient) trailing updates.  Measured: N=16384 512
+// (157 ms; 768: 159), N=24576 768 (339 ms; 512: 346), N=32768 1024 (665 ms;
+// 512: 698, 768: 678).
 #ifndef ATLG_LA_NB
-#define ATLG_LA_NB 512
+#define ATLG_LA_NB 0
 #endif
-constexpr long kLaNB = ATLG_LA_NB;
+inline long la_block(long mn) {
+  if (ATLG_LA_NB > 0) return ATLG_LA_NB;
+  return std::min(1024L, std::max(512L, mn / 32 / 128 * 128));
+}
 constexpr long kLaMin = 2048;
 
 // ---------------------------------------------------------------- helpers
@@ -1244,7 +1253,8 @@ void apply_block_lists(LU<S>& f, long k0, long kb, long c0, long n0, long c1, lo
   ATLG_LAUNCHED();
 }
 
-// Right-looking LU by kLaNB-wide block columns with one step of look-ahead.
+// Right-looking LU by la_block(min(M,N))-wide block columns with one step of
+// look-ahead.
 // Block k is factored by ATLAS's recursion (getrf_rec) with its leaves'
 // interchanges confined to the block; every leaf keeps its composed list.
 // Step k then, on the caller's stream:
@@ -1288,6 +1298,7 @@ void getrf_lookahead(LU<S>& f0) {
   f1.leaf_lists = lists;
   // block 0 on the caller's stream
   f0.win_lo = 0;
+  const long kLaNB = la_block(mn);
   f0.win_hi = std::min(kLaNB, mn);
   getrf_rec(f0, 0, f0.win_hi);
   for (long k0 = 0; k0 < mn; k0 += kLaNB) {

@@ -139,6 +139,43 @@ def test_getrf_concurrent_host_threads(G):
             assert torch.equal(outs[t][1], alone[t][1])
 
 
+@pytest.mark.parametrize("n", [24576, 32768])
+def test_getrf_wide_lookahead_blocks(G, n):
+    # orders whose look-ahead blocks are 768 and 1024 columns wide, checked
+    # by size-independent properties on the GPU: lutst's scaled residual
+    # ||P A - L U||_1 / (||A||_1 n eps) <= 100 (L U and P A formed with
+    # torch in FP64), pivots in range, and |l_ij| <= 1 (partial pivoting
+    # takes the largest remaining entry of each column)
+    import torch
+    g = torch.Generator(device="cuda")
+    g.manual_seed(n)
+    t = torch.empty((n, n), dtype=torch.float64, device="cuda").uniform_(-1, 1, generator=g)
+    w = t.clone()  # row-major t is the column-major matrix t^T
+    ipiv = torch.zeros(n, dtype=torch.int32, device="cuda")
+    info = torch.zeros(1, dtype=torch.int32, device="cuda")
+    G.dgetrf(n, n, w, n, ipiv, info)
+    torch.cuda.synchronize()
+    assert int(info.item()) == 0
+    p = ipiv.cpu().numpy()
+    assert np.all(p >= np.arange(n)) and np.all(p < n)
+    f = w.t()
+    low = torch.tril(f, -1)
+    assert float(low.abs().max()) <= 1.0
+    low.diagonal().fill_(1.0)
+    up = torch.triu(f)
+    del w, f
+    lu = torch.matmul(low, up)
+    del low, up
+    perm = np.arange(n)
+    for i in range(n):
+        perm[i], perm[p[i]] = perm[p[i]], perm[i]
+    a = t.t()
+    lu.sub_(a[torch.from_numpy(perm).cuda()])
+    r = float(lu.abs().sum(dim=0).max()) / (float(a.abs().sum(dim=0).max()) * n * 2.0 ** -53)
+    print("getrf n=%d: lutst ratio %.3g" % (n, r))
+    assert r <= 100.0
+
+
 def test_getrf_known_pivots(G):
     # A = P^T L U with |L_ij| < 0.45/n and diag(U) in [1, 2]: at every step the
     # unit-diagonal row beats every competitor by a factor > n (and L, U are
