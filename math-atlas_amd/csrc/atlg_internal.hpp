@@ -89,7 +89,9 @@ enum Slot : int {
   kSlotBf16 = 14,       // BF16 operand copies of the low-precision update
   kSlotTrsv = 15,       // ready flags of the few-RHS triangular solve
   kSlotLaLists = 16,    // per-leaf interchange lists of the look-ahead LU
-  kNumSlots = 17
+  kSlotLaPerm = 17,     // row maps of the look-ahead LU's deferred interchanges
+  kSlotLaTemp = 18,     // staging of one column block for those interchanges
+  kNumSlots = 19
 };
 void* scratch(hipStream_t s, Slot slot, size_t bytes);
 

@@ -523,6 +523,61 @@ laswp_lists(S* __restrict__ A, long lda, long c0, long n0, long c1, long n1,
   }
 }
 
+// Deferred interchanges of the look-ahead LU.  Q_b (b = 1 .. nblk-1) is
+// the row map of block b's interchanges applied to the identity: each of
+// its leaf lists ("row dst receives row src") in order.  One workgroup per
+// block; Q holds nblk-1 arrays of M rows (Q_b at (b-1)*M).
+__global__ void __launch_bounds__(256)
+la_block_maps(int M, int NB, int mn, const int* __restrict__ lists, int* __restrict__ Q) {
+  __shared__ int tmp[2 * PW];
+  const int b = blockIdx.x + 1, t = threadIdx.x;
+  int* q = Q + (size_t)blockIdx.x * M;
+  for (int i = t; i < M; i += blockDim.x) q[i] = i;
+  __syncthreads();
+  const int l0 = b * NB / PW, l1 = (min(mn, (b + 1) * NB) + PW - 1) / PW;
+  for (int l = l0; l < l1; l++) {
+    const int* lst = lists + (size_t)l * kLeafListInts;
+    const int T = lst[0];
+    if (t < T) tmp[t] = q[lst[2 + 2 * t]];
+    __syncthreads();
+    if (t < T) q[lst[1 + 2 * t]] = tmp[t];
+    __syncthreads();
+  }
+}
+
+// P_j = Q_{j+1} o P_{j+1}, P_{nblk-1} = identity: the source row of row i
+// in column block j once the interchanges of every later block are applied.
+// P holds nblk-1 arrays of M rows (P_j at j*M).
+__global__ void la_suffix_maps(int M, int nblk, const int* __restrict__ Q, int* __restrict__ P) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= M) return;
+  int p = i;
+  for (int j = nblk - 2; j >= 0; j--) {
+    p = Q[(size_t)j * M + p];  // Q_{j+1}
+    P[(size_t)j * M + i] = p;
+  }
+}
+
+// T(i - r, c) = A(P[i], c0 + c) for rows i in [r, M): one workgroup per
+// column (the column's reads stay in one XCD's L2), then copied back.
+template <class S>
+__global__ void __launch_bounds__(256)
+la_gather_rows(int M, int r, S* __restrict__ A, long lda, long c0, const int* __restrict__ P,
+               S* __restrict__ T) {
+  const long c = blockIdx.x;
+  const S* a = A + (c0 + c) * lda;
+  S* tc = T + c * (long)(M - r);
+  for (int i = r + threadIdx.x; i < M; i += blockDim.x) tc[i - r] = a[P[i]];
+}
+template <class S>
+__global__ void __launch_bounds__(256)
+la_copy_rows(int M, int r, S* __restrict__ A, long lda, long c0, const S* __restrict__ T) {
+  const long c = blockIdx.x;
+  S* a = A + (c0 + c) * lda;
+  const S* tc = T + c * (long)(M - r);
+  for (int i = r + threadIdx.x; i < M; i += blockDim.x) a[i] = tc[i - r];
+}
+
 // ---------------------------------------------------------------- panels
 
 // Words of the column mailboxes, handled as raw bits: a slot holding the
@@ -1253,6 +1308,34 @@ void apply_block_lists(LU<S>& f, long k0, long kb, long c0, long n0, long c1, lo
   ATLG_LAUNCHED();
 }
 
+// Applies, to every column block j but the last, the interchanges of all
+// later blocks (deferred there from the look-ahead steps): one composed
+// row map per block, each column permuted once through a staging buffer.
+template <class S>
+void apply_deferred_left(LU<S>& f, long NB, long mn) {
+  const long M = f.M, nblk = (mn + NB - 1) / NB;
+  if (nblk < 2) return;
+  int* Q = static_cast<int*>(scratch(f.s, kSlotLaPerm, sizeof(int) * 2 * (nblk - 1) * M));
+  int* P = Q + (nblk - 1) * M;
+  S* T = static_cast<S*>(scratch(f.s, kSlotLaTemp, sizeof(S) * M * NB));
+  hipLaunchKernelGGL(la_block_maps, dim3((unsigned)(nblk - 1)), dim3(256), 0, f.s, (int)M,
+                     (int)NB, (int)mn, f.leaf_lists, Q);
+  ATLG_LAUNCHED();
+  hipLaunchKernelGGL(la_suffix_maps, dim3((unsigned)((M + 255) / 256)), dim3(256), 0, f.s,
+                     (int)M, (int)nblk, Q, P);
+  ATLG_LAUNCHED();
+  for (long j = 0; j + 1 < nblk; j++) {
+    const long r = (j + 1) * NB;
+    if (r >= M) break;
+    hipLaunchKernelGGL(la_gather_rows<S>, dim3((unsigned)NB), dim3(256), 0, f.s, (int)M, (int)r,
+                       f.A, f.lda, j * NB, P + j * M, T);
+    ATLG_LAUNCHED();
+    hipLaunchKernelGGL(la_copy_rows<S>, dim3((unsigned)NB), dim3(256), 0, f.s, (int)M, (int)r,
+                       f.A, f.lda, j * NB, T);
+    ATLG_LAUNCHED();
+  }
+}
+
 // Right-looking LU by la_block(min(M,N))-wide block columns with one step of
 // look-ahead.
 // Block k is factored by ATLAS's recursion (getrf_rec) with its leaves'
@@ -1260,8 +1343,9 @@ void apply_block_lists(LU<S>& f, long k0, long kb, long c0, long n0, long c1, lo
 // Step k then, on the caller's stream:
 //   1. brings block k+1 up to date: block k's interchanges, U = L_kk^-1 A,
 //      and the GEMM update (the critical path);
-//   2. applies block k's interchanges to the columns left of block k and
-//      right of block k+1, and their TRSM and GEMM update;
+//   2. applies block k's interchanges to the columns right of block k+1, and
+//      their TRSM and GEMM update (the columns left of block k receive all
+//      later interchanges at the end, composed into one row map per block);
 //   3. hands block k+1 to the auxiliary stream, which factors it while 2
 //      runs (it waits for 1 only);
 //   4. waits for the panel before step k+1.
@@ -1313,7 +1397,7 @@ void getrf_lookahead(LU<S>& f0) {
     // The rest of step k is queued before the panel's many small launches,
     // so s0 has work while the host issues them.
     const long r0 = k1 + nb1, rest = N - r0;
-    apply_block_lists(f0, k0, kb, 0, k0, r0, rest);
+    apply_block_lists(f0, k0, kb, r0, rest, 0, 0);
     if (rest > 0) {
       node_trsm_cols(f0, k0, kb, r0, rest);
       if (M > k1) {
@@ -1331,6 +1415,7 @@ void getrf_lookahead(LU<S>& f0) {
       ATLG_HIP(hipStreamWaitEvent(s0, e_panel, 0));
     }
   }
+  apply_deferred_left(f0, kLaNB, mn);
   if (split) {  // every panel was joined into s0 above
     ATLG_HIP(hipEventRecord(aux_event(3), s0));
     ATLG_HIP(hipStreamWaitEvent(caller, aux_event(3), 0));
