@@ -458,6 +458,53 @@ __device__ void compose_swaps(const int* pa, const int* pb, int k, int* val, int
   if (s == 0) out[0] = *cnt;
 }
 
+// The same composition for k <= 64 interchanges by one wave, without
+// barriers: lane s holds step s; the scans over the other steps read their
+// rows from registers (readlane), the pointer jumping exchanges through
+// cross-lane permutes, and the list is written in (step, a before b) order.
+// Called by all 64 lanes of the wave.
+__device__ void compose_swaps_wave(const int* pa, const int* pb, int k, int* __restrict__ out) {
+  const int s = threadIdx.x & 63;
+  const bool act = s < k;
+  const int a = act ? pa[s] : -1, b = act ? pb[s] : -1;
+  int ra = -1, rb = -1;  // latest earlier step touching b / a, as 2q (its pa) or 2q+1 (its pb)
+  bool la = act, lb = act && a != b;  // no later step touches a / b
+  for (int q = 0; q < k; q++) {
+    const int aq = __builtin_amdgcn_readlane(a, q), bq = __builtin_amdgcn_readlane(b, q);
+    if (q < s) {
+      if (aq == b) ra = 2 * q;
+      else if (bq == b) ra = 2 * q + 1;
+      if (aq == a) rb = 2 * q;
+      else if (bq == a) rb = 2 * q + 1;
+    } else if (q > s) {
+      if (aq == a || bq == a) la = false;
+      if (bq == b || aq == b) lb = false;
+    }
+  }
+  int v0 = !act ? 0 : (ra >= 0 ? -(ra + 1) : b);
+  int v1 = !act ? 0 : (rb >= 0 ? -(rb + 1) : a);
+  for (int span = 1; span < 2 * k; span *= 2) {  // pointer jumping
+    const int i0 = v0 < 0 ? -v0 - 1 : 0, i1 = v1 < 0 ? -v1 - 1 : 0;
+    const int x00 = __shfl(v0, i0 >> 1), x01 = __shfl(v1, i0 >> 1);
+    const int x10 = __shfl(v0, i1 >> 1), x11 = __shfl(v1, i1 >> 1);
+    if (v0 < 0) v0 = (i0 & 1) ? x01 : x00;
+    if (v1 < 0) v1 = (i1 & 1) ? x11 : x10;
+  }
+  const bool e0 = la && v0 != a, e1 = lb && v1 != b;
+  const unsigned long long m0 = __ballot(e0), m1 = __ballot(e1);
+  const unsigned long long lt = (1ull << s) - 1ull;
+  const int pos = __popcll(m0 & lt) + __popcll(m1 & lt);
+  if (e0) {
+    out[1 + 2 * pos] = a;
+    out[2 + 2 * pos] = v0;
+  }
+  if (e1) {
+    out[1 + 2 * (pos + e0)] = b;
+    out[2 + 2 * (pos + e0)] = v1;
+  }
+  if (s == 0) out[0] = __popcll(m0) + __popcll(m1);
+}
+
 // Interchanges ipiv[i], i in [k1, k2) (ascending if inci > 0, else
 // descending; at most KC of them) composed into a row list.
 __global__ void __launch_bounds__(KC)
@@ -709,7 +756,7 @@ panel_coop(int M, int w, int off, S* __restrict__ P, long lda, int* __restrict__
   __shared__ int si[CP_THREADS / 64];
   __shared__ double gk[2];  // best candidate of workgroups 0..63, 64..127
   __shared__ int gi[2];
-  __shared__ int pa[PW], pb[PW], cval[2 * PW], ccnt;
+  __shared__ int pa[PW], pb[PW];
   __shared__ int zcol;  // first column with a zero pivot (workgroup 0), -1: none
   const int t = threadIdx.x, G = gridDim.x, GT = G * CP_THREADS;
   const int g = blockIdx.x * CP_THREADS + t;
@@ -955,7 +1002,7 @@ panel_coop(int M, int w, int off, S* __restrict__ P, long lda, int* __restrict__
     __syncthreads();
     if (t < mn) ipiv[off + t] = pb[t];
     if (t == 0 && zcol >= 0 && *dinfo == 0) *dinfo = off + zcol + 1;
-    compose_swaps(pa, pb, mn, cval, &ccnt, list);
+    if (t < 64) compose_swaps_wave(pa, pb, mn, list);
   }
 }
 
