@@ -137,15 +137,39 @@ __device__ void update_block(int n, S* __restrict__ A, int lda, int J, int w) {
   }
 }
 
+// D (pitch kDP) = the lower triangle of A[J:J+w, J:J+w], zero padded to
+// T x T, by wave 0: four loads per lane in flight at a time (one at a time
+// costs a memory round trip per element row when the device is busy).
+template <class S, int T>
+__device__ __forceinline__ void load_diag(const S* __restrict__ A, int lda, int J, int w, S* D) {
+  constexpr int NF = T * T / 64, NBT = 4;
+  static_assert(NF % NBT == 0, "fill batches");
+  const int l = threadIdx.x & 63;
+#pragma unroll 1
+  for (int u0 = 0; u0 < NF; u0 += NBT) {
+    S v[NBT];
+#pragma unroll
+    for (int u = 0; u < NBT; u++) {
+      const int e = l + 64 * (u0 + u), i = e % T, c = e / T;
+      v[u] = (i < w && c <= i) ? A[(J + i) + (size_t)(J + c) * lda] : S(0);
+    }
+#pragma unroll
+    for (int u = 0; u < NBT; u++) {
+      const int e = l + 64 * (u0 + u), i = e % T, c = e / T;
+      D[i * kDP + c] = v[u];
+    }
+  }
+}
+
 // ---- F: factor the w x w diagonal block (wave 0) and invert it ------------
 // Returns 0, or j+1 for the first non-positive pivot.  D (pitch kDP) receives
 // L_JJ, V its inverse, zero padded to T x T; L_JJ is written back to A.
 template <class S, int T>
 __device__ int factor_diag(int n, S* __restrict__ A, int lda, int J, int w, S* D, S* V) {
   const int l = threadIdx.x;  // wave 0 only
+  load_diag<S, T>(A, lda, J, w, D);
   for (int e = l; e < T * T; e += 64) {
     const int i = e % T, c = e / T;
-    D[i * kDP + c] = (i < w && c <= i) ? A[(J + i) + (size_t)(J + c) * lda] : S(0);
     V[i * kDP + c] = (i == c) ? S(1) : S(0);
   }
   wave_sync();
@@ -246,10 +270,7 @@ __device__ void solve_sys(int n, const S* __restrict__ L, int lda, S* __restrict
   for (int J = 0; J < n; J += T) {
     const int w = min(T, n - J);
     if (t < 64) {
-      for (int e = l; e < T * T; e += 64) {
-        const int a = e % T, c = e / T;
-        D[a * kDP + c] = (a < w && c <= a) ? L[(J + a) + (size_t)(J + c) * lda] : S(0);
-      }
+      load_diag<S, T>(L, lda, J, w, D);
       wave_sync();
       for (int r0 = 0; r0 < nrhs; r0 += RG) {
         const int r = r0 + g;
@@ -284,10 +305,7 @@ __device__ void solve_sys(int n, const S* __restrict__ L, int lda, S* __restrict
   for (int J = ((n - 1) / T) * T; J >= 0; J -= T) {
     const int w = min(T, n - J);
     if (t < 64) {
-      for (int e = l; e < T * T; e += 64) {
-        const int a = e % T, c = e / T;
-        D[a * kDP + c] = (a < w && c <= a) ? L[(J + a) + (size_t)(J + c) * lda] : S(0);
-      }
+      load_diag<S, T>(L, lda, J, w, D);
       wave_sync();
       for (int r0 = 0; r0 < nrhs; r0 += RG) {
         const int r = r0 + g;
