@@ -257,6 +257,35 @@ __device__ int chol_sys(int n, S* __restrict__ A, int lda, Smem& sm) {
   return 0;
 }
 
+// acc[q] += sum_{c < w} p[c*st] * b[c + (r0+q)*ldb] for the right-hand sides
+// r0 + q < nrhs, in order of c, four loads of p in flight at a time (used
+// for FP32 only: in FP64 the extra registers cost the kernel an occupancy
+// level).
+template <class S>
+__device__ __forceinline__ void row_dot4(const S* p, size_t st, int w, const S* b, int ldb, int r0,
+                                         int nrhs, S (&acc)[4]) {
+  int c = 0;
+  {
+#pragma unroll 1
+  for (; c + 4 <= w; c += 4) {
+    S lv[4];
+#pragma unroll
+    for (int u = 0; u < 4; u++) lv[u] = p[(size_t)(c + u) * st];
+#pragma unroll
+    for (int u = 0; u < 4; u++)
+#pragma unroll
+      for (int q = 0; q < 4; q++)
+        if (r0 + q < nrhs) acc[q] += lv[u] * b[(c + u) + (size_t)(r0 + q) * ldb];
+  }
+  }
+  for (; c < w; c++) {
+    const S lv = p[(size_t)c * st];
+#pragma unroll
+    for (int q = 0; q < 4; q++)
+      if (r0 + q < nrhs) acc[q] += lv * b[c + (size_t)(r0 + q) * ldb];
+  }
+}
+
 // Solves L L^T X = B in place (B: n x nrhs, column-major, ldb).
 template <class S, int T>
 __device__ void solve_sys(int n, const S* __restrict__ L, int lda, S* __restrict__ B, int ldb,
@@ -288,11 +317,15 @@ __device__ void solve_sys(int n, const S* __restrict__ L, int lda, S* __restrict
     for (int row = J + w + t; row < n; row += kBT) {
       for (int r0 = 0; r0 < nrhs; r0 += 4) {
         S acc[4] = {0, 0, 0, 0};
-        for (int c = 0; c < w; c++) {
-          const S lv = L[row + (size_t)(J + c) * lda];
+        if constexpr (sizeof(S) == 4) {
+          row_dot4<S>(L + row + (size_t)J * lda, (size_t)lda, w, B + J, ldb, r0, nrhs, acc);
+        } else {
+          for (int c = 0; c < w; c++) {
+            const S lv = L[row + (size_t)(J + c) * lda];
 #pragma unroll
-          for (int q = 0; q < 4; q++)
-            if (r0 + q < nrhs) acc[q] += lv * B[(J + c) + (size_t)(r0 + q) * ldb];
+            for (int q = 0; q < 4; q++)
+              if (r0 + q < nrhs) acc[q] += lv * B[(J + c) + (size_t)(r0 + q) * ldb];
+          }
         }
 #pragma unroll
         for (int q = 0; q < 4; q++)
@@ -324,11 +357,15 @@ __device__ void solve_sys(int n, const S* __restrict__ L, int lda, S* __restrict
       const S* lc = L + J + (size_t)k * lda;  // L[J:J+w, k], contiguous
       for (int r0 = 0; r0 < nrhs; r0 += 4) {
         S acc[4] = {0, 0, 0, 0};
-        for (int c = 0; c < w; c++) {
-          const S lv = lc[c];
+        if constexpr (sizeof(S) == 4) {
+          row_dot4<S>(lc, 1, w, B + J, ldb, r0, nrhs, acc);
+        } else {
+          for (int c = 0; c < w; c++) {
+            const S lv = lc[c];
 #pragma unroll
-          for (int q = 0; q < 4; q++)
-            if (r0 + q < nrhs) acc[q] += lv * B[(J + c) + (size_t)(r0 + q) * ldb];
+            for (int q = 0; q < 4; q++)
+              if (r0 + q < nrhs) acc[q] += lv * B[(J + c) + (size_t)(r0 + q) * ldb];
+          }
         }
 #pragma unroll
         for (int q = 0; q < 4; q++)
