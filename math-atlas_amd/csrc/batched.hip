@@ -259,8 +259,8 @@ __device__ int chol_sys(int n, S* __restrict__ A, int lda, Smem& sm) {
 
 // acc[q] += sum_{c < w} p[c*st] * b[c + (r0+q)*ldb] for the right-hand sides
 // r0 + q < nrhs, in order of c, four loads of p in flight at a time (used
-// for FP32 only: in FP64 the extra registers cost the kernel an occupancy
-// level).
+// by the mixed solver's kernels; the FP64 solver keeps one load at a time,
+// since the extra registers would cost its kernel an occupancy level).
 template <class S>
 __device__ __forceinline__ void row_dot4(const S* p, size_t st, int w, const S* b, int ldb, int r0,
                                          int nrhs, S (&acc)[4]) {
@@ -457,19 +457,10 @@ __device__ void residual(const MixDesc& d, int nrhs, Smem& sm, bool* conv, doubl
     double xm[4] = {0, 0, 0, 0}, rm[4] = {0, 0, 0, 0};
     for (int i = threadIdx.x; i < n; i += kBT) {
       double acc[4] = {0, 0, 0, 0};
-      for (int j = 0; j <= i; j++) {  // row i of the lower triangle
-        const double a = d.A[i + (size_t)j * d.lda];
-#pragma unroll
-        for (int q = 0; q < 4; q++)
-          if (r0 + q < nrhs) acc[q] += a * d.X[j + (size_t)(r0 + q) * d.ldx];
-      }
-      const double* col = d.A + (size_t)i * d.lda;
-      for (int j = i + 1; j < n; j++) {  // column i below the diagonal
-        const double a = col[j];
-#pragma unroll
-        for (int q = 0; q < 4; q++)
-          if (r0 + q < nrhs) acc[q] += a * d.X[j + (size_t)(r0 + q) * d.ldx];
-      }
+      // row i of the lower triangle, then column i below the diagonal
+      row_dot4<double>(d.A + i, (size_t)d.lda, i + 1, d.X, d.ldx, r0, nrhs, acc);
+      row_dot4<double>(d.A + (size_t)i * d.lda + i + 1, 1, n - i - 1, d.X + i + 1, d.ldx, r0,
+                       nrhs, acc);
 #pragma unroll
       for (int q = 0; q < 4; q++)
         if (r0 + q < nrhs) {
