@@ -492,15 +492,39 @@ mixed_posv_kernel(int nsys, const MixDesc* __restrict__ desc, int nrhs, int iter
     double rs = 0.0;
     bool over = false;
     for (int i = threadIdx.x; i < n; i += kBT) {
+      // row i of the lower triangle (converted to A32), then column i below
+      // the diagonal; four loads in flight, sums in order of j
       double s = 0.0;
-      for (int j = 0; j <= i; j++) {
+      int j = 0;
+#pragma unroll 1
+      for (; j + 4 <= i + 1; j += 4) {
+        double av[4];
+#pragma unroll
+        for (int u = 0; u < 4; u++) av[u] = d.A[i + (size_t)(j + u) * d.lda];
+#pragma unroll
+        for (int u = 0; u < 4; u++) {
+          s += fabs(av[u]);
+          if (fabs(av[u]) > (double)FLT_MAX) over = true;
+          d.A32[i + (size_t)(j + u) * n] = (float)av[u];
+        }
+      }
+      for (; j <= i; j++) {
         const double a = d.A[i + (size_t)j * d.lda];
         s += fabs(a);
         if (fabs(a) > (double)FLT_MAX) over = true;
         d.A32[i + (size_t)j * n] = (float)a;
       }
       const double* col = d.A + (size_t)i * d.lda;
-      for (int j = i + 1; j < n; j++) s += fabs(col[j]);
+      j = i + 1;
+#pragma unroll 1
+      for (; j + 4 <= n; j += 4) {
+        double av[4];
+#pragma unroll
+        for (int u = 0; u < 4; u++) av[u] = col[j + u];
+#pragma unroll
+        for (int u = 0; u < 4; u++) s += fabs(av[u]);
+      }
+      for (; j < n; j++) s += fabs(col[j]);
       rs = fmax(rs, s);
     }
     const double anrm = block_max(rs, sm);
