@@ -85,6 +85,7 @@ struct alignas(16) Smem {
   double D[kTmax * kDP];    // diagonal block (factor)
   double V[kTmax * kDP];    // its inverse
   double red[kBW * 4];      // reductions
+  double bs[kTmax * 4];     // a solve's current block of solved values (first 4 right-hand sides)
   int flag;                 // failure index + 1, 0 = fine
   int cur;                  // system taken from the queue
 };
@@ -262,10 +263,9 @@ __device__ int chol_sys(int n, S* __restrict__ A, int lda, Smem& sm) {
 // by the mixed solver's kernels; the FP64 solver keeps one load at a time,
 // since the extra registers would cost its kernel an occupancy level).
 template <class S>
-__device__ __forceinline__ void row_dot4(const S* p, size_t st, int w, const S* b, int ldb, int r0,
-                                         int nrhs, S (&acc)[4]) {
+__device__ __forceinline__ void row_dot4(const S* p, size_t st, int w, const S* b, int sc, int sq,
+                                         int r0, int nrhs, S (&acc)[4]) {
   int c = 0;
-  {
 #pragma unroll 1
   for (; c + 4 <= w; c += 4) {
     S lv[4];
@@ -275,14 +275,25 @@ __device__ __forceinline__ void row_dot4(const S* p, size_t st, int w, const S* 
     for (int u = 0; u < 4; u++)
 #pragma unroll
       for (int q = 0; q < 4; q++)
-        if (r0 + q < nrhs) acc[q] += lv[u] * b[(c + u) + (size_t)(r0 + q) * ldb];
-  }
+        if (r0 + q < nrhs) acc[q] += lv[u] * b[(size_t)(c + u) * sc + (size_t)q * sq];
   }
   for (; c < w; c++) {
     const S lv = p[(size_t)c * st];
 #pragma unroll
     for (int q = 0; q < 4; q++)
-      if (r0 + q < nrhs) acc[q] += lv * b[c + (size_t)(r0 + q) * ldb];
+      if (r0 + q < nrhs) acc[q] += lv * b[(size_t)c * sc + (size_t)q * sq];
+  }
+}
+
+// The same one load at a time (the FP64 solver; see above).
+template <class S>
+__device__ __forceinline__ void row_dot1(const S* p, size_t st, int w, const S* b, int sc, int sq,
+                                         int r0, int nrhs, S (&acc)[4]) {
+  for (int c = 0; c < w; c++) {
+    const S lv = p[(size_t)c * st];
+#pragma unroll
+    for (int q = 0; q < 4; q++)
+      if (r0 + q < nrhs) acc[q] += lv * b[(size_t)c * sc + (size_t)q * sq];
   }
 }
 
@@ -292,6 +303,7 @@ __device__ void solve_sys(int n, const S* __restrict__ L, int lda, S* __restrict
                           int nrhs, Smem& sm) {
   if (n <= 0) return;
   S* D = reinterpret_cast<S*>(sm.D);
+  S* BS = reinterpret_cast<S*>(sm.bs);  // solved block, [c][q] for q < 4
   const int t = threadIdx.x, l = t & 63;
   constexpr int RG = 64 / T;  // right-hand sides per wave pass
   const int i = l % T, g = l / T;
@@ -311,21 +323,22 @@ __device__ void solve_sys(int n, const S* __restrict__ L, int lda, S* __restrict
           else if (i > j) y -= D[i * kDP + j] * yj;
         }
         if (act) B[(J + i) + (size_t)r * ldb] = y;
+        if (act && r < 4) BS[i * 4 + r] = y;
       }
     }
     __syncthreads();
     for (int row = J + w + t; row < n; row += kBT) {
       for (int r0 = 0; r0 < nrhs; r0 += 4) {
         S acc[4] = {0, 0, 0, 0};
-        if constexpr (sizeof(S) == 4) {
-          row_dot4<S>(L + row + (size_t)J * lda, (size_t)lda, w, B + J, ldb, r0, nrhs, acc);
+        const S* lr = L + row + (size_t)J * lda;
+        // the block's solved values: from LDS for the first four right-hand sides
+        if (r0 == 0) {
+          if constexpr (sizeof(S) == 4) row_dot4<S>(lr, (size_t)lda, w, BS, 4, 1, r0, nrhs, acc);
+          else row_dot1<S>(lr, (size_t)lda, w, BS, 4, 1, r0, nrhs, acc);
         } else {
-          for (int c = 0; c < w; c++) {
-            const S lv = L[row + (size_t)(J + c) * lda];
-#pragma unroll
-            for (int q = 0; q < 4; q++)
-              if (r0 + q < nrhs) acc[q] += lv * B[(J + c) + (size_t)(r0 + q) * ldb];
-          }
+          const S* bj = B + J + (size_t)r0 * ldb;
+          if constexpr (sizeof(S) == 4) row_dot4<S>(lr, (size_t)lda, w, bj, 1, ldb, r0, nrhs, acc);
+          else row_dot1<S>(lr, (size_t)lda, w, bj, 1, ldb, r0, nrhs, acc);
         }
 #pragma unroll
         for (int q = 0; q < 4; q++)
@@ -350,6 +363,7 @@ __device__ void solve_sys(int n, const S* __restrict__ L, int lda, S* __restrict
           else if (i < j) x -= D[j * kDP + i] * xj;
         }
         if (act) B[(J + i) + (size_t)r * ldb] = x;
+        if (act && r < 4) BS[i * 4 + r] = x;
       }
     }
     __syncthreads();
@@ -357,15 +371,13 @@ __device__ void solve_sys(int n, const S* __restrict__ L, int lda, S* __restrict
       const S* lc = L + J + (size_t)k * lda;  // L[J:J+w, k], contiguous
       for (int r0 = 0; r0 < nrhs; r0 += 4) {
         S acc[4] = {0, 0, 0, 0};
-        if constexpr (sizeof(S) == 4) {
-          row_dot4<S>(lc, 1, w, B + J, ldb, r0, nrhs, acc);
+        if (r0 == 0) {
+          if constexpr (sizeof(S) == 4) row_dot4<S>(lc, 1, w, BS, 4, 1, r0, nrhs, acc);
+          else row_dot1<S>(lc, 1, w, BS, 4, 1, r0, nrhs, acc);
         } else {
-          for (int c = 0; c < w; c++) {
-            const S lv = lc[c];
-#pragma unroll
-            for (int q = 0; q < 4; q++)
-              if (r0 + q < nrhs) acc[q] += lv * B[(J + c) + (size_t)(r0 + q) * ldb];
-          }
+          const S* bj = B + J + (size_t)r0 * ldb;
+          if constexpr (sizeof(S) == 4) row_dot4<S>(lc, 1, w, bj, 1, ldb, r0, nrhs, acc);
+          else row_dot1<S>(lc, 1, w, bj, 1, ldb, r0, nrhs, acc);
         }
 #pragma unroll
         for (int q = 0; q < 4; q++)
@@ -458,8 +470,9 @@ __device__ void residual(const MixDesc& d, int nrhs, Smem& sm, bool* conv, doubl
     for (int i = threadIdx.x; i < n; i += kBT) {
       double acc[4] = {0, 0, 0, 0};
       // row i of the lower triangle, then column i below the diagonal
-      row_dot4<double>(d.A + i, (size_t)d.lda, i + 1, d.X, d.ldx, r0, nrhs, acc);
-      row_dot4<double>(d.A + (size_t)i * d.lda + i + 1, 1, n - i - 1, d.X + i + 1, d.ldx, r0,
+      const double* xr = d.X + (size_t)r0 * d.ldx;
+      row_dot4<double>(d.A + i, (size_t)d.lda, i + 1, xr, 1, d.ldx, r0, nrhs, acc);
+      row_dot4<double>(d.A + (size_t)i * d.lda + i + 1, 1, n - i - 1, xr + i + 1, 1, d.ldx, r0,
                        nrhs, acc);
 #pragma unroll
       for (int q = 0; q < 4; q++)
