@@ -80,10 +80,16 @@ __device__ __forceinline__ void wave_sync() {
   __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
-// LDS of one workgroup.  `D` and `V` are typed per scalar at use.
+// LDS of one workgroup.  `D` and `V` are typed per scalar at use; the
+// mixed solver's residual uses the same space as `XS`.
 struct alignas(16) Smem {
-  double D[kTmax * kDP];    // diagonal block (factor)
-  double V[kTmax * kDP];    // its inverse
+  union {
+    struct {
+      double D[kTmax * kDP];  // diagonal block (factor)
+      double V[kTmax * kDP];  // its inverse
+    };
+    double XS[2 * kTmax * kDP];  // residual: X[0:n, r0:r0+4] as [row][q]
+  };
   double red[kBW * 4];      // reductions
   double bs[kTmax * 4];     // a solve's current block of solved values (first 4 right-hand sides)
   int flag;                 // failure index + 1, 0 = fine
@@ -465,15 +471,31 @@ __device__ double block_max(double v, Smem& sm) {
 // right-hand side r0..r0+3 across the workgroup, writing float(r) to W32.
 __device__ void residual(const MixDesc& d, int nrhs, Smem& sm, bool* conv, double cte) {
   const int n = d.n;
+  // X staged in LDS when it fits (n <= 528): the rows' dot products then
+  // read it there instead of from global memory
+  const bool stage = 4 * n <= 2 * kTmax * kDP;
   for (int r0 = 0; r0 < nrhs; r0 += 4) {
     double xm[4] = {0, 0, 0, 0}, rm[4] = {0, 0, 0, 0};
+    if (stage) {
+      for (int e = threadIdx.x; e < 4 * n; e += kBT) {
+        const int j = e >> 2, q = e & 3;
+        sm.XS[e] = (r0 + q < nrhs) ? d.X[j + (size_t)(r0 + q) * d.ldx] : 0.0;
+      }
+      __syncthreads();
+    }
     for (int i = threadIdx.x; i < n; i += kBT) {
       double acc[4] = {0, 0, 0, 0};
       // row i of the lower triangle, then column i below the diagonal
-      const double* xr = d.X + (size_t)r0 * d.ldx;
-      row_dot4<double>(d.A + i, (size_t)d.lda, i + 1, xr, 1, d.ldx, r0, nrhs, acc);
-      row_dot4<double>(d.A + (size_t)i * d.lda + i + 1, 1, n - i - 1, xr + i + 1, 1, d.ldx, r0,
-                       nrhs, acc);
+      if (stage) {
+        row_dot4<double>(d.A + i, (size_t)d.lda, i + 1, sm.XS, 4, 1, r0, nrhs, acc);
+        row_dot4<double>(d.A + (size_t)i * d.lda + i + 1, 1, n - i - 1, sm.XS + 4 * (i + 1), 4,
+                         1, r0, nrhs, acc);
+      } else {
+        const double* xr = d.X + (size_t)r0 * d.ldx;
+        row_dot4<double>(d.A + i, (size_t)d.lda, i + 1, xr, 1, d.ldx, r0, nrhs, acc);
+        row_dot4<double>(d.A + (size_t)i * d.lda + i + 1, 1, n - i - 1, xr + i + 1, 1, d.ldx, r0,
+                         nrhs, acc);
+      }
 #pragma unroll
       for (int q = 0; q < 4; q++)
         if (r0 + q < nrhs) {
