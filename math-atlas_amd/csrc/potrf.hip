@@ -249,7 +249,15 @@ void potrf_rec(hipStream_t s, bool lower, long N, double* A, long lda, int* dinf
 #endif
 constexpr long kCholLaMin = 2048;
 
-// Right-looking Cholesky by RB-column blocks with one step of look-ahead:
+// Look-ahead step width: the chain of diagonal blocks (three 128-block
+// leaves and a panel solve per 384 columns) is the longer path at small
+// orders, the trailing SYRK at large ones.  Measured (ms): N=4096 256: 7.9,
+// 384: 8.4; N=8192 128: 15.7, 256: 16.9, 384: 18.1; N=12288 256: 30.7,
+// 384: 31.3; N=16384 256: 55, 384: 52.
+inline long chol_step(long N) { return N < 10240 ? 128 : (N < 14336 ? 256 : 384); }
+
+// Right-looking Cholesky by chol_step(N)-column blocks with one step of
+// look-ahead:
 // while the caller's stream applies step k's SYRK to the columns right of
 // block k+1, the auxiliary stream factors block k+1 (updated first) and
 // solves its panel.  lower: A = L L^T, panel L21 = A21 L11^-T, update
@@ -297,11 +305,12 @@ void potrf_lookahead(hipStream_t s0, bool lower, long N, double* A, long lda, in
            cap);
     }
   };
-  const long kb0 = std::min(RB, N);
+  const long rb = chol_step(N);
+  const long kb0 = std::min(rb, N);
   potrf_rec(s0, lower, kb0, A, lda, dinfo, 0, W);
   solve_panel(s0, 0, kb0, N - kb0);
-  for (long k0 = 0; k0 + kb0 < N; k0 += RB) {
-    const long kb = std::min(RB, N - k0), k1 = k0 + kb, nb1 = std::min(RB, N - k1);
+  for (long k0 = 0; k0 + kb0 < N; k0 += rb) {
+    const long kb = std::min(rb, N - k0), k1 = k0 + kb, nb1 = std::min(rb, N - k1);
     const long rest = N - k1 - nb1;
     // block k+1 first (all its rows), then the rest of step k's update,
     // beside which the auxiliary stream factors block k+1 and its panel
