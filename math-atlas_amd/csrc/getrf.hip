@@ -542,14 +542,16 @@ laswp_apply(long N, S* __restrict__ A, long lda, const int* __restrict__ lst) {
 
 // Applies the composed lists of consecutive leaves (lists[l], l < nl, each
 // kLeafListInts ints, in order) to the columns [c0, c0 + n0) and
-// [c1, c1 + n1): per list, gather the source rows into LDS, then scatter.
-// The same workgroup reads what the previous list wrote, so the barrier
-// between lists is enough.
+// [c1, c1 + n1): per list, every thread reads its (at most NE) source
+// elements into registers, all loads in flight, then after a barrier
+// writes them to their destinations.  The same workgroup reads what the
+// previous list wrote, so the barrier between lists is enough.
 template <class S>
 __global__ void __launch_bounds__(256)
 laswp_lists(S* __restrict__ A, long lda, long c0, long n0, long c1, long n1,
             const int* __restrict__ lists, int nl) {
-  __shared__ S buf[2 * PW * CW];
+  constexpr int NE = 2 * PW * CW / 256;  // a list moves at most 2*PW rows
+  static_assert(NE * 256 == 2 * PW * CW, "list elements per thread");
   const long nb0 = (n0 + CW - 1) / CW, b = blockIdx.x;
   const long cb = b < nb0 ? c0 + b * CW : c1 + (b - nb0) * CW;
   const long ce = b < nb0 ? c0 + n0 : c1 + n1;
@@ -557,14 +559,17 @@ laswp_lists(S* __restrict__ A, long lda, long c0, long n0, long c1, long n1,
   for (int l = 0; l < nl; l++) {
     const int* lst = lists + (size_t)l * kLeafListInts;
     const int T = lst[0];
-    for (int e = threadIdx.x; e < T * CW; e += blockDim.x) {
-      const int t = e / CW, c = e % CW;
-      if (c < nc) buf[e] = A[(size_t)lst[2 + 2 * t] + (size_t)(cb + c) * lda];
+    S v[NE];
+#pragma unroll
+    for (int u = 0; u < NE; u++) {
+      const int e = threadIdx.x + 256 * u, t = e / CW, c = e % CW;
+      if (e < T * CW && c < nc) v[u] = A[(size_t)lst[2 + 2 * t] + (size_t)(cb + c) * lda];
     }
     __syncthreads();
-    for (int e = threadIdx.x; e < T * CW; e += blockDim.x) {
-      const int t = e / CW, c = e % CW;
-      if (c < nc) A[(size_t)lst[1 + 2 * t] + (size_t)(cb + c) * lda] = buf[e];
+#pragma unroll
+    for (int u = 0; u < NE; u++) {
+      const int e = threadIdx.x + 256 * u, t = e / CW, c = e % CW;
+      if (e < T * CW && c < nc) A[(size_t)lst[1 + 2 * t] + (size_t)(cb + c) * lda] = v[u];
     }
     __syncthreads();
   }
@@ -1138,18 +1143,14 @@ S* at(const LU<S>& f, long i, long j) { return f.A + i + j * f.lda; }
 // Applies a composed list to the window's columns except [c0, c0 + w).
 template <class S>
 void apply_outside(LU<S>& f, long c0, long w, const int* lst) {
-  const long nl = c0 - f.win_lo;
-  if (nl > 0) {
-    hipLaunchKernelGGL(laswp_apply<S>, dim3((unsigned)((nl + CW - 1) / CW)), dim3(256), 0, f.s, nl,
-                       at(f, 0, f.win_lo), f.lda, lst);
-    ATLG_LAUNCHED();
-  }
-  const long r0 = c0 + w, nr = f.win_hi - r0;
-  if (nr > 0) {
-    hipLaunchKernelGGL(laswp_apply<S>, dim3((unsigned)((nr + CW - 1) / CW)), dim3(256), 0, f.s, nr,
-                       at(f, 0, r0), f.lda, lst);
-    ATLG_LAUNCHED();
-  }
+  const long nl = std::max(0L, c0 - f.win_lo);
+  const long r0 = c0 + w, nr = std::max(0L, f.win_hi - r0);
+  const long nblk = (nl + CW - 1) / CW + (nr + CW - 1) / CW;
+  if (nblk == 0) return;
+  // both column ranges in one launch
+  hipLaunchKernelGGL(laswp_lists<S>, dim3((unsigned)nblk), dim3(256), 0, f.s, f.A, f.lda,
+                     f.win_lo, nl, r0, nr, lst, 1);
+  ATLG_LAUNCHED();
 }
 
 template <class S, int RPT>
