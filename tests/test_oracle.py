@@ -257,6 +257,14 @@ def test_laswp_matches_numpy(orc):
         assert np.array_equal(b, a[rows])
 
 
+def test_npvts_counts_moved_rows(orc):
+    # lutst's NPVTS: entries with ipiv[i] != i (identity pivots count zero)
+    assert O.npvts(np.arange(7, dtype=np.int32)) == 0
+    ipiv = np.array([4, 1, 9, 3, 7, 5, 8, 7, 8, 9], dtype=np.int32)
+    assert O.npvts(ipiv) == int(np.count_nonzero(ipiv != np.arange(10)))
+    assert O.npvts(ipiv) == 4
+
+
 def test_lu_resid_detects_planted_error(orc):
     r = rng(10)
     a = np.asfortranarray(r.uniform(-1, 1, (20, 20)))
