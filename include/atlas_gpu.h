@@ -41,6 +41,34 @@ const char *atlg_last_error(void);
 void atlg_set_xerbla_hook(void (*hook)(int p, const char *rout,
                                        const char *msg));
 
+/* ---- Failure reporting of the bounded waits (no ATLAS counterpart) ----
+ * Two kernels wait on other workgroups of their own launch: the getrf
+ * panel (pivot-candidate mailboxes) and the few-RHS triangular solve
+ * (per-block ready flags).  Every such wait is bounded; a wait that runs
+ * past its bound makes the kernel exit early and ORs one of the bits below
+ * into the stream's failure word (the results of that call are then
+ * unspecified; getrf also writes *dinfo = INT_MIN and identity pivots for
+ * the affected panel, so later launches stay in bounds).
+ * atlg_stream_status: synchronizes `stream`, stores its failure word in
+ *   *status (0: no failure) and clears it.
+ * atlg_set_failure_hook: the host-pointer entry points (cblas_, clapack_,
+ *   ATL_) check the failure word of their stream before returning and
+ *   report a failure through hook(routine, bits, message); the default
+ *   (NULL) prints the message and aborts, so no host call returns a partial
+ *   result silently.  If a hook returns, getrf/potrf/gesv/posv return
+ *   INT_MIN and the outputs are unspecified.  atlg_dsgesv (which
+ *   synchronizes) returns ATLG_EWAIT instead.
+ * atlg_set_wait_limit: polls (each one memory round trip) before a wait
+ *   gives up, process-wide; polls <= 0 restores the defaults (2^20 for the
+ *   panel, 2^22 for the solve, far beyond any wait of a resident grid). */
+#define ATLG_EWAIT (-1001)
+#define ATLG_FAIL_PANEL 1
+#define ATLG_FAIL_TRSV 2
+int atlg_stream_status(atlg_stream_t stream, int *status);
+void atlg_set_failure_hook(void (*hook)(const char *rout, int bits,
+                                        const char *msg));
+void atlg_set_wait_limit(long polls);
+
 /* C <- alpha*op(A)*op(B) + beta*C; beta == 0 never reads C. */
 int atlg_dgemm(int TA, int TB, long M, long N, long K, double alpha,
                const double *A, long lda, const double *B, long ldb,

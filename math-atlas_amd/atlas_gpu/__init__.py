@@ -14,6 +14,7 @@ so element (i, j) is t[j, i]; `colmajor` and `from_fortran` build them.
 """
 import ctypes
 import os
+import threading
 
 import numpy as np
 
@@ -27,6 +28,8 @@ Upper, Lower = 121, 122
 NonUnit, Unit = 131, 132
 Left, Right = 141, 142
 EHIP = -1000
+EWAIT = -1001
+FAIL_PANEL, FAIL_TRSV = 1, 2
 
 _dp = ctypes.POINTER(ctypes.c_double)
 _ip = ctypes.POINTER(ctypes.c_int)
@@ -34,6 +37,7 @@ _i, _l, _d, _v = ctypes.c_int, ctypes.c_long, ctypes.c_double, ctypes.c_void_p
 _ul = ctypes.c_ulong
 
 XERBLA_HOOK = ctypes.CFUNCTYPE(None, ctypes.c_int, ctypes.c_char_p, ctypes.c_char_p)
+FAILURE_HOOK = ctypes.CFUNCTYPE(None, ctypes.c_char_p, ctypes.c_int, ctypes.c_char_p)
 
 _SIGS = {
     # drop-in host API
@@ -62,6 +66,9 @@ _SIGS = {
     "atlg_init": (_i, []),
     "atlg_last_error": (ctypes.c_char_p, []),
     "atlg_set_xerbla_hook": (None, [XERBLA_HOOK]),
+    "atlg_stream_status": (_i, [_v, _ip]),
+    "atlg_set_failure_hook": (None, [FAILURE_HOOK]),
+    "atlg_set_wait_limit": (None, [_l]),
     "atlg_dgemm": (_i, [_i, _i, _l, _l, _l, _d, _v, _l, _v, _l, _d, _v, _l, _v]),
     "atlg_dsyrk": (_i, [_i, _i, _l, _l, _d, _v, _l, _d, _v, _l, _v]),
     "atlg_dtrsm": (_i, [_i, _i, _i, _i, _l, _l, _d, _v, _l, _v, _l, _v]),
@@ -106,6 +113,9 @@ def lib():
             f = getattr(L, name)
             f.restype = res
             f.argtypes = args
+        # bounded-wait failures of the host entry points raise AtlgWaitError
+        # (on the calling thread) instead of the C default, print and abort
+        L.atlg_set_failure_hook(_FAILURE_CB)
         _LIB = L
     return _LIB
 
@@ -118,9 +128,62 @@ class AtlgError(RuntimeError):
     pass
 
 
+class AtlgWaitError(AtlgError):
+    """A bounded wait inside a kernel gave up (ATLG_FAIL_* bits in .bits);
+    the results of the call are unspecified."""
+
+    def __init__(self, msg, bits):
+        super().__init__(msg)
+        self.bits = bits
+
+
+_failures = threading.local()
+
+
+def _on_failure(rout, bits, msg):
+    _failures.last = (rout.decode(), bits, msg.decode())
+
+
+_FAILURE_CB = FAILURE_HOOK(_on_failure)
+
+
+def _host(fn, *args):
+    """Calls a host-pointer entry point; raises AtlgWaitError when its
+    device work reported a bounded-wait failure."""
+    _failures.last = None
+    r = fn(*args)
+    f = getattr(_failures, "last", None)
+    if f is not None:
+        _failures.last = None
+        raise AtlgWaitError("%s: %s" % (f[0], f[2]), f[1])
+    return r
+
+
+def set_wait_limit(polls):
+    """Polls before a bounded wait gives up (<= 0: the defaults)."""
+    lib().atlg_set_wait_limit(int(polls))
+
+
+def stream_status(stream=None):
+    """Synchronizes the stream and returns (and clears) its failure bits."""
+    st = ctypes.c_int(0)
+    _check(lib().atlg_stream_status(_stream(stream), ctypes.byref(st)), "atlg_stream_status")
+    return st.value
+
+
+def check_stream(stream=None):
+    """Raises AtlgWaitError if work queued on the stream reported a
+    bounded-wait failure (synchronizes the stream)."""
+    bits = stream_status(stream)
+    if bits:
+        raise AtlgWaitError("a bounded wait gave up on this stream (bits %d)" % bits, bits)
+
+
 def _check(rc, what):
     if rc == EHIP:
         raise AtlgError("%s: HIP failure: %s" % (what, lib().atlg_last_error().decode()))
+    if rc == EWAIT:
+        raise AtlgWaitError("%s: %s" % (what, lib().atlg_last_error().decode()), 0)
     if rc != 0:
         raise AtlgError("%s: illegal argument %d" % (what, -rc))
 
@@ -161,59 +224,59 @@ def _pi(a):
 
 
 def cblas_dgemm(order, ta, tb, m, n, k, alpha, a, lda, b, ldb, beta, c, ldc):
-    lib().cblas_dgemm(order, ta, tb, m, n, k, alpha, _p(a), lda, _p(b), ldb, beta, _p(c), ldc)
+    _host(lib().cblas_dgemm, order, ta, tb, m, n, k, alpha, _p(a), lda, _p(b), ldb, beta, _p(c), ldc)
 
 
 def cblas_dtrsm(order, side, uplo, ta, diag, m, n, alpha, a, lda, b, ldb):
-    lib().cblas_dtrsm(order, side, uplo, ta, diag, m, n, alpha, _p(a), lda, _p(b), ldb)
+    _host(lib().cblas_dtrsm, order, side, uplo, ta, diag, m, n, alpha, _p(a), lda, _p(b), ldb)
 
 
 def cblas_dsyrk(order, uplo, trans, n, k, alpha, a, lda, beta, c, ldc):
-    lib().cblas_dsyrk(order, uplo, trans, n, k, alpha, _p(a), lda, beta, _p(c), ldc)
+    _host(lib().cblas_dsyrk, order, uplo, trans, n, k, alpha, _p(a), lda, beta, _p(c), ldc)
 
 
 def clapack_dgetrf(order, m, n, a, lda, ipiv):
-    return lib().clapack_dgetrf(order, m, n, _p(a), lda, _pi(ipiv))
+    return _host(lib().clapack_dgetrf, order, m, n, _p(a), lda, _pi(ipiv))
 
 
 def clapack_dgetrs(order, trans, n, nrhs, a, lda, ipiv, b, ldb):
-    return lib().clapack_dgetrs(order, trans, n, nrhs, _p(a), lda, _pi(ipiv), _p(b), ldb)
+    return _host(lib().clapack_dgetrs, order, trans, n, nrhs, _p(a), lda, _pi(ipiv), _p(b), ldb)
 
 
 def clapack_dgesv(order, n, nrhs, a, lda, ipiv, b, ldb):
-    return lib().clapack_dgesv(order, n, nrhs, _p(a), lda, _pi(ipiv), _p(b), ldb)
+    return _host(lib().clapack_dgesv, order, n, nrhs, _p(a), lda, _pi(ipiv), _p(b), ldb)
 
 
 def clapack_dpotrf(order, uplo, n, a, lda):
-    return lib().clapack_dpotrf(order, uplo, n, _p(a), lda)
+    return _host(lib().clapack_dpotrf, order, uplo, n, _p(a), lda)
 
 
 def clapack_dpotrs(order, uplo, n, nrhs, a, lda, b, ldb):
-    return lib().clapack_dpotrs(order, uplo, n, nrhs, _p(a), lda, _p(b), ldb)
+    return _host(lib().clapack_dpotrs, order, uplo, n, nrhs, _p(a), lda, _p(b), ldb)
 
 
 def clapack_dposv(order, uplo, n, nrhs, a, lda, b, ldb):
-    return lib().clapack_dposv(order, uplo, n, nrhs, _p(a), lda, _p(b), ldb)
+    return _host(lib().clapack_dposv, order, uplo, n, nrhs, _p(a), lda, _p(b), ldb)
 
 
 def ATL_dlaswp(n, a, lda, k1, k2, ipiv, inci):
-    lib().ATL_dlaswp(n, _p(a), lda, k1, k2, _pi(ipiv), inci)
+    _host(lib().ATL_dlaswp, n, _p(a), lda, k1, k2, _pi(ipiv), inci)
 
 
 def ATL_dgemm(ta, tb, m, n, k, alpha, a, lda, b, ldb, beta, c, ldc):
-    lib().ATL_dgemm(ta, tb, m, n, k, alpha, _p(a), lda, _p(b), ldb, beta, _p(c), ldc)
+    _host(lib().ATL_dgemm, ta, tb, m, n, k, alpha, _p(a), lda, _p(b), ldb, beta, _p(c), ldc)
 
 
 def ATL_dtrsm(side, uplo, ta, diag, m, n, alpha, a, lda, b, ldb):
-    lib().ATL_dtrsm(side, uplo, ta, diag, m, n, alpha, _p(a), lda, _p(b), ldb)
+    _host(lib().ATL_dtrsm, side, uplo, ta, diag, m, n, alpha, _p(a), lda, _p(b), ldb)
 
 
 def ATL_dgetrf(order, m, n, a, lda, ipiv):
-    return lib().ATL_dgetrf(order, m, n, _p(a), lda, _pi(ipiv))
+    return _host(lib().ATL_dgetrf, order, m, n, _p(a), lda, _pi(ipiv))
 
 
 def ATL_dpotrf(order, uplo, n, a, lda):
-    return lib().ATL_dpotrf(order, uplo, n, _p(a), lda)
+    return _host(lib().ATL_dpotrf, order, uplo, n, _p(a), lda)
 
 
 # ---------------------------------------------------------------- device API
@@ -269,9 +332,13 @@ def dsyrk(uplo, trans, n, k, alpha, a, lda, beta, c, ldc, stream=None):
                             _stream(stream)), "atlg_dsyrk")
 
 
-def dtrsm(side, uplo, ta, diag, m, n, alpha, a, lda, b, ldb, stream=None):
+def dtrsm(side, uplo, ta, diag, m, n, alpha, a, lda, b, ldb, stream=None, check=False):
+    """check=True: synchronize and raise AtlgWaitError on a bounded-wait
+    failure (the few-RHS solve)."""
     _check(lib().atlg_dtrsm(side, uplo, ta, diag, m, n, alpha, _ptr(a), lda, _ptr(b), ldb,
                             _stream(stream)), "atlg_dtrsm")
+    if check:
+        check_stream(stream)
 
 
 def dlaswp(n, a, lda, k1, k2, ipiv, inci, stream=None):
@@ -279,14 +346,20 @@ def dlaswp(n, a, lda, k1, k2, ipiv, inci, stream=None):
            "atlg_dlaswp")
 
 
-def dgetrf(m, n, a, lda, ipiv, info, stream=None):
+def dgetrf(m, n, a, lda, ipiv, info, stream=None, check=False):
+    """check=True: synchronize and raise AtlgWaitError on a bounded-wait
+    failure of a panel (info then holds INT_MIN)."""
     _check(lib().atlg_dgetrf(m, n, _ptr(a), lda, _ptr(ipiv), _ptr(info), _stream(stream)),
            "atlg_dgetrf")
+    if check:
+        check_stream(stream)
 
 
-def dgetrs(trans, n, nrhs, a, lda, ipiv, b, ldb, stream=None):
+def dgetrs(trans, n, nrhs, a, lda, ipiv, b, ldb, stream=None, check=False):
     _check(lib().atlg_dgetrs(trans, n, nrhs, _ptr(a), lda, _ptr(ipiv), _ptr(b), ldb,
                              _stream(stream)), "atlg_dgetrs")
+    if check:
+        check_stream(stream)
 
 
 def dpotrf(uplo, n, a, lda, info, stream=None):
@@ -400,9 +473,11 @@ def sgemm_bf16(m, n, k, alpha, a, lda, b, ldb, beta, c, ldc, split=0, stream=Non
                                  split, _stream(stream)), "atlg_sgemm_bf16")
 
 
-def sgetrf(m, n, a, lda, ipiv, info, gemm_prec=0, stream=None):
+def sgetrf(m, n, a, lda, ipiv, info, gemm_prec=0, stream=None, check=False):
     _check(lib().atlg_sgetrf(m, n, _ptr(a), lda, _ptr(ipiv), _ptr(info), gemm_prec,
                              _stream(stream)), "atlg_sgetrf")
+    if check:
+        check_stream(stream)
 
 
 def sgetrs(trans, n, nrhs, a, lda, ipiv, b, ldb, stream=None):

@@ -9,11 +9,13 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <climits>
 #include <cstdarg>
 #include <cstdio>
 #include <cstdlib>
 #include <mutex>
 #include <string>
+#include <type_traits>
 
 #include "atlas_gpu_atl.h"
 #include "atlas_gpu_clapack.h"
@@ -219,11 +221,36 @@ void h_potrs(int Order, int Uplo, long N, long NRHS, const double* A, long lda, 
   ATLG_HIP(hipStreamSynchronize(s));
 }
 
+// After a host call's device work: reads and clears the library stream's
+// failure word and reports a bounded wait that gave up through the failure
+// hook.  True when there was a failure (and the hook returned).
+bool check_failure(const char* rout) {
+  const int st = take_status(lib_stream());
+  if (st == 0) return false;
+  std::string msg = "a bounded wait gave up:";
+  if (st & ATLG_FAIL_PANEL) msg += " getrf panel pivot exchange;";
+  if (st & ATLG_FAIL_TRSV) msg += " few-RHS triangular solve ready flags;";
+  msg += " the results of this call are unspecified";
+  report_failure(rout, st, msg.c_str());
+  return true;
+}
+
+// Runs a host entry point under the API mutex: HIP failures abort with a
+// message, and a bounded-wait failure goes to the failure hook; if the hook
+// returns, routines with an int result return INT_MIN.
 template <class F>
 auto locked(const char* rout, F&& f) -> decltype(f()) {
   std::lock_guard<std::mutex> lk(g_api_mu);
   try {
-    return f();
+    if constexpr (std::is_void_v<decltype(f())>) {
+      f();
+      check_failure(rout);
+      return;
+    } else {
+      const auto r = f();
+      if (check_failure(rout)) return INT_MIN;
+      return r;
+    }
   } catch (const HipFailure& h) {
     fail_hip(h, rout);
   }

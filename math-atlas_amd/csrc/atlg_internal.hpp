@@ -95,6 +95,20 @@ enum Slot : int {
 };
 void* scratch(hipStream_t s, Slot slot, size_t bytes);
 
+// The stream's failure word: a device int that kernels whose bounded waits
+// give up OR their ATLG_FAIL_* bit into (zero otherwise).  Allocated and
+// cleared on first use per stream; read and cleared by take_status().
+int* status_word(hipStream_t s);
+// Copies the stream's failure word to the host, waits for the stream,
+// clears the word and returns its value.
+int take_status(hipStream_t s);
+// Poll bound of the bounded waits: the process-wide value set by
+// atlg_set_wait_limit, or `dflt` when none is set.
+long wait_limit(long dflt);
+// Reports a failure of a host entry point through the failure hook
+// (atlg_set_failure_hook); the default hook prints and aborts.
+void report_failure(const char* rout, int status, const char* msg);
+
 // ---- kernels / drivers (device pointers, column-major) --------------------
 
 // GEMM family (S = double on the FP64 MFMA, S = float on the FP32 MFMA).

@@ -23,8 +23,9 @@
 //    column's write-once mailbox; every workgroup polls the mailbox slots
 //    until all are written (one round trip, no barrier), picks the same
 //    global pivot, swaps, scales and applies the rank-1 update to its rows.
-//    The waits are bounded; on overrun every workgroup leaves and
-//    *dinfo = INT_MIN reports the failure.
+//    The waits are bounded; on overrun every workgroup leaves, the panel's
+//    pivots are written as the identity, *dinfo = INT_MIN and the stream's
+//    failure word gets ATLG_FAIL_PANEL (checked by the host entry points).
 //
 // Pivot choice follows cblas_idamax (AtlasBase/Antoine/atlrefblas.base:753+):
 // first index of the largest |a|, NaN never chosen; the column is scaled by
@@ -66,7 +67,7 @@ constexpr int KC = 256;           // interchanges composed per laswp chunk
 constexpr int CW = 16;            // columns per laswp workgroup
 constexpr int kLeafListInts = 1 + 4 * PW;  // one leaf's composed list: count + pairs
 constexpr int kJrowT = 128;  // first thread that gathers row j in the panel exchange
-constexpr long kSpinLimit = 1L << 20;  // mailbox polls (each one round trip) before giving up
+constexpr long kSpinLimit = 1L << 20;  // default mailbox polls (each one round trip) before giving up
 // Fewest panel rows per thread of the cooperative panel (1, 2 or 4): more
 // rows per thread means fewer workgroups exchanging candidates per column.
 #ifndef ATLG_PANEL_RPT_MIN
@@ -540,6 +541,127 @@ laswp_apply(long N, S* __restrict__ A, long lda, const int* __restrict__ lst) {
   }
 }
 
+// ---- ATL_dlaswp as one composed permutation ---------------------------
+//
+// The interchanges are cut into chunks of KC; every chunk is composed into
+// its own "row i receives row j" list in parallel (one workgroup each), the
+// lists are folded in order into one row map held in LDS, and every column
+// is then permuted in one pass: its touched row range read once into LDS
+// (coalesced), written back once in the new order (coalesced).  Used while
+// the leading dimension fits the LDS staging (kSwapRows rows).
+constexpr int kSwapRows = 20224;          // rows staged per column (158 KiB of doubles)
+constexpr int kSwapRowsSmall = 4096;      // staging of short columns (more workgroups per CU)
+constexpr int kChunkInts = 1 + 4 * KC;    // one chunk's composed list
+
+// Chunk c of the interchanges [k1, k2) in application order.
+__device__ inline void swap_chunk(int c, int k1, int k2, int inci, int& a, int& b) {
+  if (inci > 0) {
+    a = k1 + c * KC;
+    b = min(k2, a + KC);
+  } else {
+    b = k2 - c * KC;
+    a = max(k1, b - KC);
+  }
+}
+
+__global__ void __launch_bounds__(KC)
+laswp_compose_chunks(const int* __restrict__ ipiv, int k1, int k2, int inci,
+                     int* __restrict__ lists) {
+  __shared__ int pa[KC], pb[KC], val[2 * KC], cnt;
+  int a, b;
+  swap_chunk(blockIdx.x, k1, k2, inci, a, b);
+  const int s = threadIdx.x, k = b - a;
+  if (s < k) {
+    const int i = inci > 0 ? a + s : b - 1 - s;
+    pa[s] = i;
+    pb[s] = ipiv[i];
+  }
+  __syncthreads();
+  compose_swaps(pa, pb, k, val, &cnt, lists + (size_t)blockIdx.x * kChunkInts);
+}
+
+// Folds the chunk lists, in order, into the row map of rows [0, R): map[r]
+// = the original row whose content ends in row r.  hdr[0..1] receives the
+// range [lo, hi) of rows that change (lo >= hi: none).
+__global__ void __launch_bounds__(1024)
+laswp_build_map(int nchunk, int R, const int* __restrict__ lists, int* __restrict__ map,
+                int* __restrict__ hdr) {
+  __shared__ int smap[kSwapRows];
+  __shared__ int slo[16], shi[16];
+  const int t = threadIdx.x;
+  for (int r = t; r < R; r += 1024) smap[r] = r;
+  __syncthreads();
+  for (int c = 0; c < nchunk; c++) {
+    const int* lst = lists + (size_t)c * kChunkInts;
+    const int T = lst[0];  // <= 2 * KC <= 1024
+    const int v = t < T ? smap[lst[2 + 2 * t]] : 0;
+    __syncthreads();
+    if (t < T) smap[lst[1 + 2 * t]] = v;
+    __syncthreads();
+  }
+  int lo = INT_MAX, hi = -1;
+  for (int r = t; r < R; r += 1024) {
+    const int m = smap[r];
+    map[r] = m;
+    if (m != r) {
+      lo = min(lo, r);
+      hi = max(hi, r);
+    }
+  }
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) {
+    lo = min(lo, __shfl_xor(lo, o));
+    hi = max(hi, __shfl_xor(hi, o));
+  }
+  if ((t & 63) == 0) {
+    slo[t >> 6] = lo;
+    shi[t >> 6] = hi;
+  }
+  __syncthreads();
+  if (t == 0) {
+    for (int w = 1; w < 16; w++) {
+      lo = min(lo, slo[w]);
+      hi = max(hi, shi[w]);
+    }
+    hdr[0] = hi < 0 ? 0 : lo;
+    hdr[1] = hi + 1;
+  }
+}
+
+// One workgroup per column: rows [lo, hi) into LDS (all of a thread's loads
+// in flight together), then row r <- staged row map[r].
+template <class S, int CAP>
+__global__ void __launch_bounds__(1024)
+laswp_cols(S* __restrict__ A, long lda, const int* __restrict__ map, const int* __restrict__ hdr) {
+  __shared__ S col[CAP];
+  constexpr int U = (CAP + 1023) / 1024;
+  const int lo = hdr[0], hi = hdr[1], t = threadIdx.x;
+  if (hi <= lo) return;
+  S* a = A + (size_t)blockIdx.x * lda;
+  S v[U];
+#pragma unroll
+  for (int u = 0; u < U; u++) {
+    const int r = lo + t + 1024 * u;
+    if (r < hi) v[u] = a[r];
+  }
+#pragma unroll
+  for (int u = 0; u < U; u++) {
+    const int r = lo + t + 1024 * u;
+    if (r < hi) col[r - lo] = v[u];
+  }
+  __syncthreads();
+#pragma unroll
+  for (int u = 0; u < U; u++) {
+    const int r = lo + t + 1024 * u;
+    if (r < hi) v[u] = col[map[r] - lo];
+  }
+#pragma unroll
+  for (int u = 0; u < U; u++) {
+    const int r = lo + t + 1024 * u;
+    if (r < hi) a[r] = v[u];
+  }
+}
+
 // Applies the composed lists of consecutive leaves (lists[l], l < nl, each
 // kLeafListInts ints, in order) to the columns [c0, c0 + n0) and
 // [c1, c1 + n1): per list, every thread reads its (at most NE) source
@@ -749,7 +871,7 @@ template <class S, int RPT>
 __global__ void __launch_bounds__(CP_THREADS)
 panel_coop(int M, int w, int off, S* __restrict__ P, long lda, int* __restrict__ ipiv,
            int* __restrict__ dinfo, CoopBuf<S>* __restrict__ cb, int par, int gclear,
-           int* __restrict__ list) {
+           int* __restrict__ list, int* __restrict__ status, long limit) {
   typedef typename Word<S>::U U;
   if (ATLG_PANEL_PRIO > 0) __builtin_amdgcn_s_setprio(ATLG_PANEL_PRIO);
   __shared__ S jrow[PW];
@@ -892,8 +1014,9 @@ panel_coop(int M, int w, int off, S* __restrict__ P, long lda, int* __restrict__
           waiting |= e < G * PW && c < w && cw[u] == Word<S>::kSent;
         }
         if (!waiting) break;
-        // the error word (another workgroup gave up) is read every 256 polls
-        if ((++spins & 255) == 0 && (ld_agent(&mb->err) || spins > kSpinLimit)) {
+        // past the bound, or another workgroup gave up (its error word is
+        // read every 256 polls)
+        if (++spins >= limit || ((spins & 255) == 0 && ld_agent(&mb->err))) {
           st_agent(&mb->err, 1);
           sbad = 1;
           break;
@@ -999,8 +1122,14 @@ panel_coop(int M, int w, int off, S* __restrict__ P, long lda, int* __restrict__
     }
   }
   if (!ok) {
-    if (t == 0) *dinfo = INT_MIN;
-    if (blockIdx.x == 0 && t == 0) list[0] = 0;  // apply no interchanges
+    if (t == 0) {
+      *dinfo = INT_MIN;
+      atomicOr(status, ATLG_FAIL_PANEL);
+    }
+    if (blockIdx.x == 0) {
+      if (t < mn) ipiv[off + t] = off + t;  // identity: later users stay in bounds
+      if (t == 0) list[0] = 0;              // apply no interchanges
+    }
     return;
   }
   if (blockIdx.x == 0) {
@@ -1114,6 +1243,8 @@ struct LU {
   bool coop = true;     // cooperative launch of the panel kernel
   int gemm_cap = 0;     // grid cap of the trailing updates (0: none)
   int* leaf_lists = nullptr;  // look-ahead: leaf at column c keeps its list at c/PW
+  int* status = nullptr;      // the caller's stream's failure word
+  long limit = kSpinLimit;    // mailbox polls before a panel gives up
   bool small_lds = false;     // runs beside a GEMM: keep kernels' LDS small
   int mb_par = 0;             // mailbox the next multi-workgroup panel uses
   int mb_g[2] = {0, 0};       // grid of each mailbox's last user (0: clean)
@@ -1140,9 +1271,12 @@ inline void node_update(LU<float>& f, long m, long n, long k, const float* A, fl
 template <class S>
 S* at(const LU<S>& f, long i, long j) { return f.A + i + j * f.lda; }
 
-// Applies a composed list to the window's columns except [c0, c0 + w).
+// Applies a composed list to the window's columns except [c0, c0 + w).  The
+// list is a leaf's: w <= PW interchanges, so at most 2*PW rows, the most
+// laswp_lists holds per list.
 template <class S>
 void apply_outside(LU<S>& f, long c0, long w, const int* lst) {
+  if (w > PW) throw HipFailure{"getrf: leaf wider than the interchange list capacity"};
   const long nl = std::max(0L, c0 - f.win_lo);
   const long r0 = c0 + w, nr = std::max(0L, f.win_hi - r0);
   const long nblk = (nl + CW - 1) / CW + (nr + CW - 1) / CW;
@@ -1172,11 +1306,12 @@ void launch_coop(LU<S>& f, int M, int w, int off, int G, int* lst) {
     // waiting for a slot gets one as the independent GEMM's workgroups
     // retire; the mailbox waits are bounded anyway.
     hipLaunchKernelGGL((panel_coop<S, RPT>), dim3(G), dim3(CP_THREADS), 0, f.s, M, w, off, P, lda,
-                       f.ipiv, f.dinfo, f.cb, par, gclear, lst);
+                       f.ipiv, f.dinfo, f.cb, par, gclear, lst, f.status, f.limit);
     ATLG_LAUNCHED();
     return;
   }
-  void* args[] = {&M, &w, &off, &P, &lda, &f.ipiv, &f.dinfo, &f.cb, &par, &gclear, &lst};
+  void* args[] = {&M,      &w,      &off, &P,   &lda,      &f.ipiv,  &f.dinfo,
+                  &f.cb,   &par,    &gclear, &lst, &f.status, &f.limit};
   ATLG_HIP(hipLaunchCooperativeKernel(reinterpret_cast<const void*>(&panel_coop<S, RPT>), dim3(G),
                                       dim3(CP_THREADS), args, 0, f.s));
 }
@@ -1286,6 +1421,29 @@ void set_int(hipStream_t s, int* p, int v) {
 template <class S>
 void laswp(hipStream_t s, long N, S* A, long lda, int K1, int K2, const int* ipiv, int inci) {
   if (N <= 0 || K2 <= K1) return;
+  // every row index (K1..K2 and the pivots) is below lda: one composed
+  // permutation, one pass over the columns
+  if (lda <= kSwapRows && N <= INT_MAX) {
+    const int nchunk = (K2 - K1 + KC - 1) / KC, R = (int)lda;
+    int* ws = static_cast<int*>(
+        scratch(s, kSlotLaswp, sizeof(int) * ((size_t)nchunk * kChunkInts + R + 64)));
+    int* lists = ws;
+    int* map = ws + (size_t)nchunk * kChunkInts;
+    int* hdr = map + R;
+    hipLaunchKernelGGL(laswp_compose_chunks, dim3((unsigned)nchunk), dim3(KC), 0, s, ipiv, K1, K2,
+                       inci, lists);
+    ATLG_LAUNCHED();
+    hipLaunchKernelGGL(laswp_build_map, dim3(1), dim3(1024), 0, s, nchunk, R, lists, map, hdr);
+    ATLG_LAUNCHED();
+    if (lda <= kSwapRowsSmall)
+      hipLaunchKernelGGL((laswp_cols<S, kSwapRowsSmall>), dim3((unsigned)N), dim3(1024), 0, s, A,
+                         lda, map, hdr);
+    else
+      hipLaunchKernelGGL((laswp_cols<S, kSwapRows>), dim3((unsigned)N), dim3(1024), 0, s, A, lda,
+                         map, hdr);
+    ATLG_LAUNCHED();
+    return;
+  }
   int* lst = static_cast<int*>(scratch(s, kSlotLaswp, sizeof(int) * (1 + 4 * KC)));
   const unsigned nb = (unsigned)((N + CW - 1) / CW);
   const int nchunk = (K2 - K1 + KC - 1) / KC;
@@ -1424,6 +1582,8 @@ void getrf_lookahead(LU<S>& f0) {
   int* lists = static_cast<int*>(scratch(s0, kSlotLaLists, sizeof(int) * kLeafListInts * nleaf));
   LU<S> f1;
   init_lu(f1, s1, f0.M, f0.N, f0.A, f0.lda, f0.ipiv, f0.dinfo, f0.gemm_prec);
+  f1.status = f0.status;  // failures are reported on the caller's stream
+  f1.limit = f0.limit;
   f1.coop = ATLG_LA_COOP != 0;
   f1.small_lds = true;
   f0.leaf_lists = lists;
@@ -1479,6 +1639,8 @@ void getrf(hipStream_t s, long M, long N, S* A, long lda, int* ipiv, int* dinfo,
   if (M == 0 || N == 0) return;
   LU<S> f;
   init_lu(f, s, M, N, A, lda, ipiv, dinfo, gemm_prec);
+  f.status = status_word(s);
+  f.limit = wait_limit(kSpinLimit);
   if (ATLG_LU_LOOKAHEAD && std::min(M, N) >= kLaMin)
     getrf_lookahead(f);
   else

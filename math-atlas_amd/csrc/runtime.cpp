@@ -8,7 +8,10 @@
 // allocate.
 #include <hip/hip_runtime.h>
 
+#include <atomic>
+#include <climits>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <map>
 #include <vector>
@@ -30,6 +33,19 @@ struct SlotBuf {
   size_t bytes = 0;
 };
 std::map<std::pair<hipStream_t, int>, SlotBuf> g_slots;
+std::map<hipStream_t, int*> g_status;  // stream -> failure word
+std::atomic<long> g_wait_limit{0};
+void (*g_failure_hook)(const char*, int, const char*) = nullptr;
+
+// Orderly release of everything the runtime created: run once at process
+// exit, registered after the HIP runtime has initialized (so it runs before
+// the runtime's own teardown).  Waits for each device, frees the scratch
+// slots and failure words, destroys the events and streams.
+void teardown();
+std::once_flag g_atexit_once;
+void arm_teardown() {
+  std::call_once(g_atexit_once, [] { std::atexit(teardown); });
+}
 }  // namespace
 
 void set_error(const std::string& msg) { g_last_error = msg; }
@@ -45,6 +61,7 @@ hipStream_t library_stream() {
   hipStream_t st;
   ATLG_HIP(hipStreamCreate(&st));
   g_streams[dev] = st;
+  arm_teardown();
   return st;
 }
 
@@ -65,6 +82,7 @@ Aux& aux_for_device() {
   if (!a.stream) {
     ATLG_HIP(hipStreamCreateWithFlags(&a.stream, hipStreamNonBlocking));
     for (auto& e : a.ev) ATLG_HIP(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+    arm_teardown();
   }
   return a;
 }
@@ -130,8 +148,87 @@ void* scratch(hipStream_t s, Slot slot, size_t bytes) {
   }
   b.p = p;
   b.bytes = want;
+  arm_teardown();
   return p;
 }
+
+int* status_word(hipStream_t s) {
+  std::lock_guard<std::mutex> lk(g_mu);
+  int*& w = g_status[s];
+  if (!w) {
+    int* p = nullptr;
+    hipError_t e = hipMalloc(&p, 256);
+    if (e == hipSuccess) e = hipMemset(p, 0, 256);
+    if (e != hipSuccess) {
+      (void)hipGetLastError();
+      if (p) (void)hipFree(p);
+      g_status.erase(s);
+      throw HipFailure{std::string("failure word: ") + hipGetErrorString(e)};
+    }
+    w = p;
+    arm_teardown();
+  }
+  return w;
+}
+
+int take_status(hipStream_t s) {
+  int* w = status_word(s);
+  int v = 0;
+  ATLG_HIP(hipMemcpyAsync(&v, w, sizeof(int), hipMemcpyDeviceToHost, s));
+  ATLG_HIP(hipStreamSynchronize(s));
+  if (v != 0) {
+    set_int(s, w, 0);
+    ATLG_HIP(hipStreamSynchronize(s));
+  }
+  return v;
+}
+
+long wait_limit(long dflt) {
+  const long v = g_wait_limit.load();
+  return v > 0 ? v : dflt;
+}
+
+void report_failure(const char* rout, int status, const char* msg) {
+  if (g_failure_hook) {
+    g_failure_hook(rout, status, msg);
+    return;
+  }
+  fprintf(stderr, "%s: %s (failure bits 0x%x)\n", rout, msg, status);
+  abort();  // never return a silent partial result
+}
+
+namespace {
+void teardown() {
+  std::lock_guard<std::mutex> lk(g_mu);
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess) return;
+  for (int d = 0; d < ndev; d++) {
+    if (hipSetDevice(d) == hipSuccess) (void)hipDeviceSynchronize();
+  }
+  for (auto& kv : g_slots)
+    if (kv.second.p) (void)hipFree(kv.second.p);
+  g_slots.clear();
+  for (auto& kv : g_status)
+    if (kv.second) (void)hipFree(kv.second);
+  g_status.clear();
+  for (auto& kv : g_aux) {
+    Aux& a = kv.second;
+    for (auto& sp : a.split) {
+      if (sp.second.first) (void)hipStreamDestroy(sp.second.first);
+      if (sp.second.second) (void)hipStreamDestroy(sp.second.second);
+    }
+    a.split.clear();
+    for (auto& e : a.ev)
+      if (e) (void)hipEventDestroy(e);
+    if (a.stream) (void)hipStreamDestroy(a.stream);
+    a.stream = nullptr;
+  }
+  g_aux.clear();
+  for (auto& kv : g_streams) (void)hipStreamDestroy(kv.second);
+  g_streams.clear();
+  (void)hipGetLastError();
+}
+}  // namespace
 
 }  // namespace atlg
 
@@ -164,6 +261,17 @@ int atlg_init(void) {
 }
 
 const char* atlg_last_error(void) { return g_last_error.c_str(); }
+
+int atlg_stream_status(atlg_stream_t stream, int* status) {
+  if (!status) return -2;
+  return guarded([&] { *status = take_status(resolve_stream(stream)); });
+}
+
+void atlg_set_failure_hook(void (*hook)(const char*, int, const char*)) {
+  g_failure_hook = hook;
+}
+
+void atlg_set_wait_limit(long polls) { g_wait_limit.store(polls); }
 
 int atlg_dgemm(int TA, int TB, long M, long N, long K, double alpha,
                const double* A, long lda, const double* B, long ldb,
@@ -371,10 +479,18 @@ int atlg_dsgesv(long N, long NRHS, double* A, long lda, int* ipiv, const double*
   if (lowprec < 0 || lowprec > 3) return -12;
   if (refine != 0 && refine != 1) return -13;
   if (!iter) return -14;
-  return guarded([&] {
-    dsgesv(resolve_stream(stream), N, NRHS, A, lda, ipiv, B, ldb, X, ldx, dinfo,
-           itermax > 0 ? itermax : 30, lowprec, refine, iter, inner);
+  int st = 0;
+  const int rc = guarded([&] {
+    hipStream_t s = resolve_stream(stream);
+    dsgesv(s, N, NRHS, A, lda, ipiv, B, ldb, X, ldx, dinfo, itermax > 0 ? itermax : 30, lowprec,
+           refine, iter, inner);
+    st = take_status(s);
   });
+  if (rc == 0 && st != 0) {
+    set_error("atlg_dsgesv: a bounded wait gave up (failure bits " + std::to_string(st) + ")");
+    return ATLG_EWAIT;
+  }
+  return rc;
 }
 
 namespace {

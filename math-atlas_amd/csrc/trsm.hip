@@ -195,13 +195,14 @@ void solve(const Ctx<S>& c, long r0, long n, S alpha) {
 // is published (a ready flag per block), then applies the block's inverse
 // from tri_inverse_blocks and publishes x.  The smallest unfinished block
 // always has all its inputs, so the launch cannot deadlock while every
-// workgroup is resident (cooperative launch); the waits are bounded anyway
-// and an overrun sets the error word.  Published x and the flags go through
+// workgroup is resident (cooperative launch); the waits are bounded anyway:
+// an overrun sets the launch's error word, which makes every workgroup
+// leave, and ORs ATLG_FAIL_TRSV into the stream's failure word.  Published x and the flags go through
 // agent-scope (coherence point) atomic loads and stores, as in getrf's
 // cooperative panel.
 constexpr int SV_T = 256;                  // threads: 2 per block row
 constexpr long kTrsvMin = 1024;            // shortest triangle that takes this path
-constexpr long kSvSpin = 1L << 22;         // polls (round trips) before giving up
+constexpr long kSvSpin = 1L << 22;         // default polls (round trips) before giving up
 
 template <class S>
 __device__ inline S ld_co(const S* p);
@@ -232,6 +233,16 @@ __device__ inline int ld_flag(const int* p) {
 __device__ inline void st_flag(int* p, int v) {
   __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
+// Publication of a solved block: the flag store is a release at agent scope,
+// and the reader's flag load an acquire, so the x values stored before the
+// flag (by every thread, ordered by the agent-scope fence and the barrier
+// in front of the flag store) are visible once the flag reads 1.
+__device__ inline void st_flag_release(int* p, int v) {
+  __hip_atomic_store(p, v, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ inline int ld_flag_acquire(const int* p) {
+  return __hip_atomic_load(const_cast<int*>(p), __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT);
+}
 
 __global__ void sv_reset(int* flags, int n) {
   for (int e = blockIdx.x * blockDim.x + threadIdx.x; e < n; e += gridDim.x * blockDim.x)
@@ -242,7 +253,7 @@ template <class S, int R>
 __global__ void __launch_bounds__(SV_T)
 trsv_sf_kernel(int n, int nb, int upper, S alpha, const S* __restrict__ A, long lda,
                const S* __restrict__ Dinv, S* __restrict__ B, long ldb, int* __restrict__ flags,
-               int* __restrict__ err) {
+               int* __restrict__ err, int* __restrict__ status, long limit) {
   __shared__ S part[2][NB][R];
   __shared__ S xs[NB][R];
   __shared__ int ok;
@@ -271,9 +282,10 @@ trsv_sf_kernel(int n, int nb, int upper, S alpha, const S* __restrict__ A, long 
         // one round trip per poll; the error word only every 256 polls
         bool good = true;
         long spins = 0;
-        while (ld_flag(&flags[q]) == 0) {
-          if ((++spins & 255) == 0 && (ld_flag(err) || spins > kSvSpin)) {
+        while (ld_flag_acquire(&flags[q]) == 0) {
+          if (++spins >= limit || ((spins & 255) == 0 && ld_flag(err))) {
             st_flag(err, 1);
+            atomicOr(status, ATLG_FAIL_TRSV);
             good = false;
             break;
           }
@@ -282,6 +294,9 @@ trsv_sf_kernel(int n, int nb, int upper, S alpha, const S* __restrict__ A, long 
       }
       __syncthreads();
       if (!ok) return;  // same value in every thread
+      // thread 0's acquire covers its own later loads; the other threads
+      // read x after the barrier, so they take the acquire side as well
+      __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
       if (t < NB) {
 #pragma unroll
         for (int k = 0; k < R; k++) xs[t][k] = (t < cols) ? ld_co(&B[c0 + t + (size_t)k * ldb]) : S(0);
@@ -325,9 +340,12 @@ trsv_sf_kernel(int n, int nb, int upper, S alpha, const S* __restrict__ A, long 
 #pragma unroll
       for (int k = 0; k < R; k++) st_co(&B[r0 + t + (size_t)k * ldb], part[0][t][k] + part[1][t][k]);
     }
-    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
-    __syncthreads();  // every store of x has completed
-    if (t == 0) st_flag(&flags[p], 1);
+    // every thread's stores of x reach the coherence point (agent-scope
+    // release: L2 writeback and a wait for the stores) before the barrier,
+    // and the flag is stored after it with release semantics
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+    __syncthreads();
+    if (t == 0) st_flag_release(&flags[p], 1);
   }
 }
 
@@ -347,7 +365,9 @@ bool trsv_few(hipStream_t s, bool lower, long n, long nrhs, S alpha, const S* A,
   ATLG_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
   int nn = (int)n, nbk = nb, up = lower ? 0 : 1;
   int G = std::max(1, std::min(nb, cus));
-  void* args[] = {&nn, &nbk, &up, &alpha, &A, &lda, &Dinv, &B, &ldb, &flags, &err};
+  int* status = status_word(s);
+  long limit = wait_limit(kSvSpin);
+  void* args[] = {&nn, &nbk, &up, &alpha, &A, &lda, &Dinv, &B, &ldb, &flags, &err, &status, &limit};
   const void* fn = nullptr;
   switch (nrhs) {
     case 1: fn = reinterpret_cast<const void*>(&trsv_sf_kernel<S, 1>); break;
