@@ -81,6 +81,25 @@ int atlg_dsyrk(int Uplo, int Trans, long N, long K, double alpha,
 int atlg_dtrsm(int Side, int Uplo, int TA, int Diag, long M, long N,
                double alpha, const double *A, long lda, double *B, long ldb,
                atlg_stream_t stream);
+/* atlg_dtrsm without any allocation: no counterpart in the reference; for
+ * callers that cannot allocate device memory (code under HIP graph capture,
+ * tight serving loops, harnesses that forbid hipMalloc).  Same arguments,
+ * semantics and results as atlg_dtrsm, except that the order of op(A)
+ * (M for Left, N for Right) may be at most 8192: larger orders return
+ * ATLG_ESIZE (atlg_last_error() says why) and queue nothing.  Its workspace
+ * (the inverses of the 128 x 128 diagonal blocks, 8 MiB) is a static device
+ * array, one per GPU, held until the process ends and shared by every call
+ * on that GPU: calls that overlap on one GPU (different streams, or host
+ * threads) race and must be serialized by the caller.  It never calls
+ * hipMalloc or hipFree (atlg_init resolves the array's address on the
+ * current device; another device's on its first call).  Few right-hand
+ * sides take the blocked solve here, not atlg_dtrsm's sync-free one, so
+ * results agree with atlg_dtrsm to rounding, bit for bit when NRHS > 4 or
+ * the order is below 1024. */
+#define ATLG_ESIZE (-1002)
+int atlg_dtrsm_noalloc(int Side, int Uplo, int TA, int Diag, long M, long N,
+                       double alpha, const double *A, long lda, double *B,
+                       long ldb, atlg_stream_t stream);
 /* Row interchanges ipiv[K1..K2) applied in order (inci=1) or in reverse
  * (inci=-1) to N columns of A. */
 int atlg_dlaswp(long N, double *A, long lda, int K1, int K2, const int *ipiv,

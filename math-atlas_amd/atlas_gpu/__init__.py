@@ -29,6 +29,7 @@ NonUnit, Unit = 131, 132
 Left, Right = 141, 142
 EHIP = -1000
 EWAIT = -1001
+ESIZE = -1002
 FAIL_PANEL, FAIL_TRSV = 1, 2
 
 _dp = ctypes.POINTER(ctypes.c_double)
@@ -72,6 +73,7 @@ _SIGS = {
     "atlg_dgemm": (_i, [_i, _i, _l, _l, _l, _d, _v, _l, _v, _l, _d, _v, _l, _v]),
     "atlg_dsyrk": (_i, [_i, _i, _l, _l, _d, _v, _l, _d, _v, _l, _v]),
     "atlg_dtrsm": (_i, [_i, _i, _i, _i, _l, _l, _d, _v, _l, _v, _l, _v]),
+    "atlg_dtrsm_noalloc": (_i, [_i, _i, _i, _i, _l, _l, _d, _v, _l, _v, _l, _v]),
     "atlg_dlaswp": (_i, [_l, _v, _l, _i, _i, _v, _i, _v]),
     "atlg_dgetrf": (_i, [_l, _l, _v, _l, _v, _v, _v]),
     "atlg_dgetrs": (_i, [_i, _l, _l, _v, _l, _v, _v, _l, _v]),
@@ -339,6 +341,16 @@ def dtrsm(side, uplo, ta, diag, m, n, alpha, a, lda, b, ldb, stream=None, check=
                             _stream(stream)), "atlg_dtrsm")
     if check:
         check_stream(stream)
+
+
+def dtrsm_noalloc(side, uplo, ta, diag, m, n, alpha, a, lda, b, ldb, stream=None):
+    """atlg_dtrsm_noalloc: dtrsm with no device allocation (order of op(A)
+    <= 8192; its static workspace is shared by every call on the GPU)."""
+    rc = lib().atlg_dtrsm_noalloc(side, uplo, ta, diag, m, n, alpha, _ptr(a), lda, _ptr(b), ldb,
+                                  _stream(stream))
+    if rc == ESIZE:
+        raise AtlgError("atlg_dtrsm_noalloc: %s" % lib().atlg_last_error().decode())
+    _check(rc, "atlg_dtrsm_noalloc")
 
 
 def dlaswp(n, a, lda, k1, k2, ipiv, inci, stream=None):

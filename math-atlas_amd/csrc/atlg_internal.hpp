@@ -146,6 +146,19 @@ void trsm(hipStream_t s, int side, int uplo, int ta, int diag, long M, long N, S
           const S* A, long lda, S* B, long ldb, const int* skip = nullptr,
           const S* dinv = nullptr, int dinv_trans = 0);
 
+// trsm() without allocation: the diagonal-block inverses live in a
+// file-scope device array sized for op(A) of order <= kNoAllocMaxOrder (one
+// copy per device, shared by every call on it, so calls that overlap on one
+// device race); the solve is the blocked recursion.  noalloc_scratch()
+// resolves the array's address on the current device (once per device).
+constexpr long kNoAllocMaxOrder = 8192;
+constexpr size_t kNoAllocBytes =
+    (size_t)(kNoAllocMaxOrder / kTrsmNB) * kTrsmNB * kTrsmNB * sizeof(double);
+template <class S>
+void trsm_noalloc(hipStream_t s, int side, int uplo, int ta, int diag, long M, long N, S alpha,
+                  const S* A, long lda, S* B, long ldb);
+void* noalloc_scratch();
+
 template <class S>
 void laswp(hipStream_t s, long N, S* A, long lda, int K1, int K2, const int* ipiv, int inci);
 

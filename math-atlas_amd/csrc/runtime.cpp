@@ -257,7 +257,10 @@ bool bad_side(int s) { return s != kLeft && s != kRight; }
 extern "C" {
 
 int atlg_init(void) {
-  return guarded([] { (void)library_stream(); });
+  return guarded([] {
+    (void)library_stream();
+    (void)noalloc_scratch();  // the allocation-free TRSM's array on this device
+  });
 }
 
 const char* atlg_last_error(void) { return g_last_error.c_str(); }
@@ -327,6 +330,27 @@ int atlg_dtrsm(int Side, int Uplo, int TA, int Diag, long M, long N,
   if (ldb < std::max(1L, M)) return -11;
   return guarded([&] {
     trsm(resolve_stream(stream), Side, Uplo, TA, Diag, M, N, alpha, A, lda, B, ldb);
+  });
+}
+
+int atlg_dtrsm_noalloc(int Side, int Uplo, int TA, int Diag, long M, long N, double alpha,
+                       const double* A, long lda, double* B, long ldb, atlg_stream_t stream) {
+  if (bad_side(Side)) return -1;
+  if (bad_uplo(Uplo)) return -2;
+  if (bad_trans(TA)) return -3;
+  if (bad_diag(Diag)) return -4;
+  if (M < 0) return -5;
+  if (N < 0) return -6;
+  if (lda < std::max(1L, Side == kLeft ? M : N)) return -9;
+  if (ldb < std::max(1L, M)) return -11;
+  const long na = Side == kLeft ? M : N;
+  if (na > kNoAllocMaxOrder) {
+    set_error("atlg_dtrsm_noalloc: order of op(A) " + std::to_string(na) +
+              " exceeds the allocation-free maximum " + std::to_string(kNoAllocMaxOrder));
+    return ATLG_ESIZE;
+  }
+  return guarded([&] {
+    trsm_noalloc(resolve_stream(stream), Side, Uplo, TA, Diag, M, N, alpha, A, lda, B, ldb);
   });
 }
 

@@ -17,6 +17,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <mutex>
 
 #include "atlg_internal.hpp"
 #include "tri_inv.hpp"
@@ -440,6 +441,44 @@ void trsm(hipStream_t s, int side, int uplo, int ta, int diag, long M, long N, S
   solve(c, 0, na, alpha);
 }
 
+void* noalloc_scratch();
+
+// The allocation-free form: the diagonal-block inverses go to the file-scope
+// device array (at most kNoAllocMaxOrder / kTrsmNB blocks), and the solve is
+// the blocked recursion of trsm() (no few-RHS launch, whose ready flags and
+// failure word would need memory of their own).
+template <class S>
+void trsm_noalloc(hipStream_t s, int side, int uplo, int ta, int diag, long M, long N, S alpha,
+                  const S* A, long lda, S* B, long ldb) {
+  if (M <= 0 || N <= 0) return;
+  if (alpha == S(0)) {
+    scale<S>(s, M, N, S(0), B, ldb, 0, nullptr);
+    return;
+  }
+  const long na = side == kLeft ? M : N;
+  if (na > kNoAllocMaxOrder) throw HipFailure{"trsm_noalloc: order above the allocation-free maximum"};
+  S* D = static_cast<S*>(noalloc_scratch());
+  tri_inverse_blocks<S>(s, na, A, lda, ta, uplo, diag, D, nullptr);
+  Ctx<S> c;
+  c.s = s;
+  c.left = (side == kLeft);
+  c.trans = is_trans(ta);
+  c.lower_eff = ((uplo == kLower) != c.trans);
+  c.A = A;
+  c.lda = lda;
+  c.B = B;
+  c.ldb = ldb;
+  c.M = M;
+  c.N = N;
+  c.skip = nullptr;
+  c.Dinv = D;
+  c.dtrans = kNoTrans;
+  solve(c, 0, na, alpha);
+}
+
+template void trsm_noalloc<double>(hipStream_t, int, int, int, int, long, long, double,
+                                   const double*, long, double*, long);
+
 template void tri_inverse_blocks<double>(hipStream_t, long, const double*, long, int, int, int,
                                          double*, const int*);
 template void tri_inverse_blocks<float>(hipStream_t, long, const float*, long, int, int, int,
@@ -448,5 +487,36 @@ template void trsm<double>(hipStream_t, int, int, int, int, long, long, double, 
                            long, double*, long, const int*, const double*, int);
 template void trsm<float>(hipStream_t, int, int, int, int, long, long, float, const float*, long,
                           float*, long, const int*, const float*, int);
+
+}  // namespace atlg
+
+// ---- the device array behind atlg_dtrsm_noalloc ---------------------------
+// kNoAllocMaxOrder / kTrsmNB inverses of 128 x 128 doubles (8 MiB), one copy
+// per device, held until the process ends.  Its name is only a host-side
+// stand-in for the symbol: host code uses the address that
+// hipGetSymbolAddress returns for the current device, never the name itself.
+static __device__ __align__(256) unsigned char scratch[atlg::kNoAllocBytes];
+
+namespace atlg {
+namespace {
+constexpr int kMaxDevices = 64;
+std::mutex g_noalloc_mu;
+void* g_noalloc_ptr[kMaxDevices] = {};
+}  // namespace
+
+// Address of the array on the current device, resolved once per device
+// (with that device current) and kept.
+void* noalloc_scratch() {
+  int dev = 0;
+  ATLG_HIP(hipGetDevice(&dev));
+  if (dev < 0 || dev >= kMaxDevices) throw HipFailure{"trsm_noalloc: device index out of range"};
+  std::lock_guard<std::mutex> lk(g_noalloc_mu);
+  if (!g_noalloc_ptr[dev]) {
+    void* p = nullptr;
+    ATLG_HIP(hipGetSymbolAddress(&p, HIP_SYMBOL(::scratch)));
+    g_noalloc_ptr[dev] = p;
+  }
+  return g_noalloc_ptr[dev];
+}
 
 }  // namespace atlg
