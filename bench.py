@@ -499,8 +499,12 @@ def main():
         "pct_fp64_peak": round(100.0 * value / PEAK_FP64_MFMA, 2),
         "roofline": {"bound": "mfma", "achieved": round(achieved, 3), "peak": PEAK_FP64_MFMA,
                      "unit": "TFLOP/s", "frac": round(achieved / PEAK_FP64_MFMA, 4),
-                     "traffic": traffic, "kernel": "dgemm_mfma<NN>",
-                     "avg_launch_ms": round(avg_launch, 4)},
+                     "traffic": traffic, "kernel": "gemm_nn_f64",
+                     "avg_launch_ms": round(avg_launch, 4),
+                     # sustained-load drift: per-launch spread and first vs last launches
+                     "min_launch_ms": round(min(per), 4), "max_launch_ms": round(max(per), 4),
+                     "first5_avg_ms": round(sum(per[:5]) / len(per[:5]), 4),
+                     "last5_avg_ms": round(sum(per[-5:]) / len(per[-5:]), 4)},
         "check_max_rel_err": rel,
         "wall_s_timed": round(wall, 3),
     }

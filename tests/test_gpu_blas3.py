@@ -97,6 +97,21 @@ def test_gemm_beta0_overwrites_nan_and_alpha0(G):
     assert nbad == 0
 
 
+def test_gemm_nn_register_path_edges(G):
+    # NoTrans x NoTrans with even leading dimensions and K a multiple of 32
+    # takes the kernel that loads A straight into registers: partial row
+    # and column tiles (odd M included), C with an odd leading dimension
+    # (scalar epilogue), beta = 0 over NaN, alpha = -1.1, and ragged grids
+    for (m, n, k, gap, alpha, beta, nan_c) in (
+            (129, 257, 64, 2, 1.0, 1.0, False), (127, 3, 32, 0, -1.1, 1.3, False),
+            (2113, 200, 96, 4, 1.0, 0.0, True), (3, 130, 32, 1, 0.7, 1.0, False),
+            (1, 1, 32, 2, 1.0, -1.1, False), (384, 384, 512, 0, 1.0, 1.0, False),
+            (255, 129, 160, 2, -1.1, 0.0, True)):
+        r, nbad, mx = gemm_case(G, O.NoTrans, O.NoTrans, m, n, k, alpha, beta, gap=gap,
+                                nan_c=nan_c)
+        assert r < 50.0 and nbad == 0, (m, n, k, gap, alpha, beta, r, nbad, mx)
+
+
 def test_gemm_config1_1000(G):
     # the CPU-runnable headline case: M=N=K=1000, alpha=beta=1
     r, nbad, mx = gemm_case(G, O.NoTrans, O.NoTrans, 1000, 1000, 1000, 1.0, 1.0)
