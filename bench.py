@@ -41,7 +41,7 @@ def parse():
     ap.add_argument("--n", type=int, default=16384)
     ap.add_argument("--no-sweep", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--cpu-cols", type=int, default=160)
+    ap.add_argument("--cpu-cols", type=int, default=160, help="CPU baseline columns per thread")
     ap.add_argument("--only", default="", help="profile helper: run only this sweep item")
     ap.add_argument("--ir-n", type=int, default=32768, help="order of the mixed-precision solve")
     ap.add_argument("--copy-calib", action="store_true",
@@ -166,7 +166,8 @@ def sweep(torch, G, only="", ir_n=32768):
         out["dpotrf_16384_info"] = int(info.item())
         del s, s0
     # the extensions report an error string instead of ending the run
-    for key, fn in (("ir", lambda: bench_ir(torch, G, ir_n)),
+    for key, fn in (("config1", lambda: bench_config1(G)),
+                    ("ir", lambda: bench_ir(torch, G, ir_n)),
                     ("batched", lambda: bench_batched(torch, G)),
                     ("cholgrad", lambda: bench_cholgrad(torch))):
         if not want(key):
@@ -393,18 +394,76 @@ def bench_cholgrad(torch, n=8192):
     return out
 
 
-def cpu_baseline(n, ncols):
+def host_cores():
+    """Threads the CPU legs use: the host's cores, capped by OMP_NUM_THREADS
+    when the environment sets it (the CPU share a GPU box grants)."""
+    n = os.cpu_count() or 1
+    env = os.environ.get("OMP_NUM_THREADS", "")
+    if env.isdigit() and int(env) > 0:
+        n = min(n, int(env))
+    return n
+
+
+def cpu_model():
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def cpu_baseline(n, ncols_per_thread):
+    """The oracle's textbook DGEMM (ATL_drefgemm loop order) on a column slab
+    of the same N=16384 problem, split over the host's cores."""
     import numpy as np
     import oracle as O
+    th = host_cores()
+    ncols = ncols_per_thread * th
     rng = np.random.default_rng(7)
     a = np.asfortranarray(rng.uniform(-1.0, 1.0, (n, n)))
     b = np.asfortranarray(rng.uniform(-1.0, 1.0, (n, ncols)))
     c = np.asfortranarray(rng.uniform(-1.0, 1.0, (n, ncols)))
-    sec = O.time_refgemm_slab(n, ncols, n, a, b, c)
+    sec = O.time_refgemm_slab_mt(n, ncols, n, a, b, c, th)
     flops = 2.0 * n * ncols * n
-    return {"value": flops / sec / 1e12, "unit": "TFLOP/s", "cores": 1, "kind": "port",
+    return {"value": flops / sec / 1e12, "unit": "TFLOP/s", "cores": th, "kind": "port",
+            "nproc": os.cpu_count(), "cpu_model": cpu_model(),
             "sample": "oracle textbook DGEMM (ATL_drefgemmNN loop order), M=K=%d, first %d "
-                      "columns of C (%.1f GFLOP in %.1f s), 1 thread" % (n, ncols, flops / 1e9, sec)}
+                      "columns of C (%.1f GFLOP in %.1f s), %d threads" % (
+                          n, ncols, flops / 1e9, sec, th)}
+
+
+def bench_config1(G):
+    """Config 1: cblas_dgemm M=N=K=1000, alpha=beta=1, U[-1,1], through the
+    host-pointer C ABI (staging to and from HBM included), checked with
+    l3blastst's residual; beside it the oracle's textbook DGEMM of the same
+    problem on the host's cores."""
+    import numpy as np
+    import oracle as O
+    n = 1000
+    rng = np.random.default_rng(1)
+    a, b, c0 = (np.asfortranarray(rng.uniform(-1.0, 1.0, (n, n))) for _ in range(3))
+    c = c0.copy(order="F")
+    G.cblas_dgemm(G.ColMajor, G.NoTrans, G.NoTrans, n, n, n, 1.0, a, n, b, n, 1.0, c, n)
+    best = None
+    for _ in range(5):
+        c = c0.copy(order="F")
+        t0 = time.perf_counter()
+        G.cblas_dgemm(G.ColMajor, G.NoTrans, G.NoTrans, n, n, n, 1.0, a, n, b, n, 1.0, c, n)
+        dt = time.perf_counter() - t0
+        best = dt if best is None else min(best, dt)
+    ref = c0.copy(order="F")
+    th = host_cores()
+    cpu = min(O.time_refgemm_slab_mt(n, n, n, a, b, ref.copy(order="F"), th) for _ in range(3))
+    O.refgemm(O.NoTrans, O.NoTrans, n, n, n, 1.0, a, b, 1.0, ref)
+    ratio = O.gemm_resid(O.NoTrans, O.NoTrans, n, n, n, 1.0, a, b, 1.0, c0, c, ref)
+    flops = 2.0 * n ** 3
+    return {"config1_cblas_dgemm_1000_ms": best * 1e3,
+            "config1_cblas_dgemm_1000_gflops": flops / best / 1e9,
+            "config1_l3blastst_ratio": ratio,
+            "config1_cpu_oracle_ms": cpu * 1e3, "config1_cpu_oracle_threads": th}
 
 
 def main():

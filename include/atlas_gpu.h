@@ -117,6 +117,27 @@ int atlg_dpotrf(int Uplo, long N, double *A, long lda, int *dinfo,
 int atlg_dpotrs(int Uplo, long N, long NRHS, const double *A, long lda,
                 double *B, long ldb, atlg_stream_t stream);
 
+/* The rest of the real level-3 BLAS and the triangular / SPD inverses,
+ * column-major on device pointers (same operations as cblas_dtrmm,
+ * cblas_dsymm, cblas_dsyr2k, ATL_dtrtri, ATL_dlauum and clapack_dpotri).
+ * atlg_dtrtri / atlg_dpotri write info to *dinfo (device): the 1-based
+ * index of the first zero on the diagonal of the triangle (A then
+ * untouched), else 0. */
+int atlg_dtrmm(int Side, int Uplo, int TA, int Diag, long M, long N,
+               double alpha, const double *A, long lda, double *B, long ldb,
+               atlg_stream_t stream);
+int atlg_dsymm(int Side, int Uplo, long M, long N, double alpha,
+               const double *A, long lda, const double *B, long ldb,
+               double beta, double *C, long ldc, atlg_stream_t stream);
+int atlg_dsyr2k(int Uplo, int Trans, long N, long K, double alpha,
+                const double *A, long lda, const double *B, long ldb,
+                double beta, double *C, long ldc, atlg_stream_t stream);
+int atlg_dtrtri(int Uplo, int Diag, long N, double *A, long lda, int *dinfo,
+                atlg_stream_t stream);
+int atlg_dlauum(int Uplo, long N, double *A, long lda, atlg_stream_t stream);
+int atlg_dpotri(int Uplo, long N, double *A, long lda, int *dinfo,
+                atlg_stream_t stream);
+
 /* ---- Adjoints of the Cholesky path (no ATLAS counterpart) -------------
  * For K = L L^T with L lower (as atlg_dpotrf with Lower leaves it):
  * atlg_dpotrf_adjoint: on entry G holds Lbar (lower triangle read, upper

@@ -58,6 +58,32 @@ void ATL_dpotrs(const enum CBLAS_ORDER Order, const enum CBLAS_UPLO Uplo,
                 const int N, const int NRHS, const double *A, const int lda,
                 double *B, const int ldb);
 
+/* replaces ATL_dtrmm, ATL_dsymm, ATL_dsyr2k (prototypes gen
+ * include/atlas_level3.h from AtlasBase/Clint/atlas.base:21021-21048) */
+void ATL_dtrmm(const enum CBLAS_SIDE Side, const enum CBLAS_UPLO Uplo,
+               const enum CBLAS_TRANSPOSE TA, const enum CBLAS_DIAG Diag,
+               ATL_CSZT M, ATL_CSZT N, const double alpha, const double *A,
+               ATL_CSZT lda, double *B, ATL_CSZT ldb);
+void ATL_dsymm(const enum CBLAS_SIDE Side, const enum CBLAS_UPLO Uplo,
+               ATL_CSZT M, ATL_CSZT N, const double alpha, const double *A,
+               ATL_CSZT lda, const double *B, ATL_CSZT ldb,
+               const double beta, double *C, ATL_CSZT ldc);
+void ATL_dsyr2k(const enum CBLAS_UPLO Uplo, const enum CBLAS_TRANSPOSE Trans,
+                ATL_CSZT N, ATL_CSZT K, const double alpha, const double *A,
+                ATL_CSZT lda, const double *B, ATL_CSZT ldb,
+                const double beta, double *C, ATL_CSZT ldc);
+/* replaces ATL_dtrtri (atlas-lp.base:2341-2380) */
+int ATL_dtrtri(const enum CBLAS_ORDER Order, const enum CBLAS_UPLO Uplo,
+               const enum CBLAS_DIAG Diag, const int N, double *A,
+               const int lda);
+/* replaces ATL_dlauum (atlas-lp.base:312-330) */
+void ATL_dlauum(const enum CBLAS_ORDER Order, const enum CBLAS_UPLO Uplo,
+                const int N, double *A, const int lda);
+/* the body of clapack_dpotri (atlas-clp.base:356-357: ATL_trtri, then
+ * ATL_lauum when it succeeded) */
+int ATL_dpotri(const enum CBLAS_ORDER Order, const enum CBLAS_UPLO Uplo,
+               const int N, double *A, const int lda);
+
 #ifdef __cplusplus
 }
 #endif

@@ -56,6 +56,33 @@ void cblas_dsyrk(const enum CBLAS_ORDER Order, const enum CBLAS_UPLO Uplo,
                  const double alpha, const double *A, const int lda,
                  const double beta, double *C, const int ldc);
 
+/* Replaces cblas_dtrmm (AtlasBase/Clint/cblas.base:587-693, the wrapper it
+ * shares with TRSM): B <- alpha*op(A)*B (Left) or alpha*B*op(A) (Right),
+ * in place; same checks and parameter numbers as cblas_dtrsm. */
+void cblas_dtrmm(const enum CBLAS_ORDER Order, const enum CBLAS_SIDE Side,
+                 const enum CBLAS_UPLO Uplo, const enum CBLAS_TRANSPOSE TA,
+                 const enum CBLAS_DIAG Diag, const int M, const int N,
+                 const double alpha, const double *A, const int lda,
+                 double *B, const int ldb);
+
+/* Replaces cblas_dsymm (AtlasBase/Clint/cblas.base:237-338):
+ * C <- alpha*A*B + beta*C (Left) or alpha*B*A + beta*C (Right), A symmetric
+ * with only the Uplo triangle referenced. */
+void cblas_dsymm(const enum CBLAS_ORDER Order, const enum CBLAS_SIDE Side,
+                 const enum CBLAS_UPLO Uplo, const int M, const int N,
+                 const double alpha, const double *A, const int lda,
+                 const double *B, const int ldb, const double beta, double *C,
+                 const int ldc);
+
+/* Replaces cblas_dsyr2k (AtlasBase/Clint/cblas.base:450-586):
+ * C_uplo <- alpha*(op(A)*op(B)^T + op(B)*op(A)^T) + beta*C_uplo, op(X) = X
+ * (NoTrans) or X^T (Trans); the other triangle is not referenced. */
+void cblas_dsyr2k(const enum CBLAS_ORDER Order, const enum CBLAS_UPLO Uplo,
+                  const enum CBLAS_TRANSPOSE Trans, const int N, const int K,
+                  const double alpha, const double *A, const int lda,
+                  const double *B, const int ldb, const double beta, double *C,
+                  const int ldc);
+
 #ifdef __cplusplus
 }
 #endif

@@ -45,6 +45,27 @@ int clapack_dposv(const enum CBLAS_ORDER Order, const enum CBLAS_UPLO Uplo,
                   const int N, const int NRHS, double *A, const int lda,
                   double *B, const int ldb);
 
+/* Replaces clapack_dtrtri (atlas-clp.base:411-452, with ATL_trtri at
+ * atlas-lp.base:2341-2380): A <- inv(A) for the Uplo triangle; a zero on a
+ * NonUnit diagonal returns its 1-based index and leaves A unchanged.  (The
+ * reference wrapper reads its error code uninitialized; here the inverse is
+ * computed exactly when every argument is legal.) */
+int clapack_dtrtri(const enum CBLAS_ORDER Order, const enum CBLAS_UPLO Uplo,
+                   const enum CBLAS_DIAG Diag, const int N, double *A,
+                   const int lda);
+
+/* Replaces clapack_dpotri (atlas-clp.base:316-362): the inverse of the SPD
+ * matrix whose Cholesky factor (clapack_dpotrf) A holds, in the Uplo
+ * triangle; info > 0 when the factor has a zero diagonal entry. */
+int clapack_dpotri(const enum CBLAS_ORDER Order, const enum CBLAS_UPLO Uplo,
+                   const int N, double *A, const int lda);
+
+/* Replaces clapack_dlauum (atlas-clp.base:316-362, ATL_lauum at
+ * atlas-lp.base:221-330): Lower A <- L^T*L, Upper A <- U*U^T, in the Uplo
+ * triangle (ColMajor; RowMajor is the other triangle of the same memory). */
+int clapack_dlauum(const enum CBLAS_ORDER Order, const enum CBLAS_UPLO Uplo,
+                   const int N, double *A, const int lda);
+
 #ifdef __cplusplus
 }
 #endif

@@ -36,6 +36,7 @@ _dp = ctypes.POINTER(ctypes.c_double)
 _ip = ctypes.POINTER(ctypes.c_int)
 _i, _l, _d, _v = ctypes.c_int, ctypes.c_long, ctypes.c_double, ctypes.c_void_p
 _ul = ctypes.c_ulong
+_cp = ctypes.c_char_p
 
 XERBLA_HOOK = ctypes.CFUNCTYPE(None, ctypes.c_int, ctypes.c_char_p, ctypes.c_char_p)
 FAILURE_HOOK = ctypes.CFUNCTYPE(None, ctypes.c_char_p, ctypes.c_int, ctypes.c_char_p)
@@ -63,6 +64,36 @@ _SIGS = {
     "ATL_dpotrfL": (_i, [_i, _dp, _i]),
     "ATL_dpotrfU": (_i, [_i, _dp, _i]),
     "ATL_dpotrs": (None, [_i, _i, _i, _i, _dp, _i, _dp, _i]),
+    "cblas_dtrmm": (None, [_i, _i, _i, _i, _i, _i, _i, _d, _dp, _i, _dp, _i]),
+    "cblas_dsymm": (None, [_i, _i, _i, _i, _i, _d, _dp, _i, _dp, _i, _d, _dp, _i]),
+    "cblas_dsyr2k": (None, [_i, _i, _i, _i, _i, _d, _dp, _i, _dp, _i, _d, _dp, _i]),
+    "clapack_dtrtri": (_i, [_i, _i, _i, _i, _dp, _i]),
+    "clapack_dpotri": (_i, [_i, _i, _i, _dp, _i]),
+    "clapack_dlauum": (_i, [_i, _i, _i, _dp, _i]),
+    "ATL_dtrmm": (None, [_i, _i, _i, _i, _ul, _ul, _d, _dp, _ul, _dp, _ul]),
+    "ATL_dsymm": (None, [_i, _i, _ul, _ul, _d, _dp, _ul, _dp, _ul, _d, _dp, _ul]),
+    "ATL_dsyr2k": (None, [_i, _i, _ul, _ul, _d, _dp, _ul, _dp, _ul, _d, _dp, _ul]),
+    "ATL_dtrtri": (_i, [_i, _i, _i, _i, _dp, _i]),
+    "ATL_dlauum": (None, [_i, _i, _i, _dp, _i]),
+    "ATL_dpotri": (_i, [_i, _i, _i, _dp, _i]),
+    # Fortran 77 interface (every argument by reference)
+    "xerbla_": (None, [ctypes.c_char_p, _ip, ctypes.c_size_t]),
+    "dgemm_": (None, [_cp, _cp, _ip, _ip, _ip, _dp, _dp, _ip, _dp, _ip, _dp, _dp, _ip]),
+    "dsyrk_": (None, [_cp, _cp, _ip, _ip, _dp, _dp, _ip, _dp, _dp, _ip]),
+    "dtrsm_": (None, [_cp, _cp, _cp, _cp, _ip, _ip, _dp, _dp, _ip, _dp, _ip]),
+    "dtrmm_": (None, [_cp, _cp, _cp, _cp, _ip, _ip, _dp, _dp, _ip, _dp, _ip]),
+    "dsymm_": (None, [_cp, _cp, _ip, _ip, _dp, _dp, _ip, _dp, _ip, _dp, _dp, _ip]),
+    "dsyr2k_": (None, [_cp, _cp, _ip, _ip, _dp, _dp, _ip, _dp, _ip, _dp, _dp, _ip]),
+    "dgetrf_": (None, [_ip, _ip, _dp, _ip, _ip, _ip]),
+    "dgetrs_": (None, [_cp, _ip, _ip, _dp, _ip, _ip, _dp, _ip, _ip]),
+    "dgesv_": (None, [_ip, _ip, _dp, _ip, _ip, _dp, _ip, _ip]),
+    "dpotrf_": (None, [_cp, _ip, _dp, _ip, _ip]),
+    "dpotrs_": (None, [_cp, _ip, _ip, _dp, _ip, _dp, _ip, _ip]),
+    "dposv_": (None, [_cp, _ip, _ip, _dp, _ip, _dp, _ip, _ip]),
+    "dtrtri_": (None, [_cp, _cp, _ip, _dp, _ip, _ip]),
+    "dpotri_": (None, [_cp, _ip, _dp, _ip, _ip]),
+    "dlauum_": (None, [_cp, _ip, _dp, _ip, _ip]),
+    "dlaswp_": (None, [_ip, _dp, _ip, _ip, _ip, _ip, _ip]),
     # device API
     "atlg_init": (_i, []),
     "atlg_last_error": (ctypes.c_char_p, []),
@@ -79,6 +110,12 @@ _SIGS = {
     "atlg_dgetrs": (_i, [_i, _l, _l, _v, _l, _v, _v, _l, _v]),
     "atlg_dpotrf": (_i, [_i, _l, _v, _l, _v, _v]),
     "atlg_dpotrs": (_i, [_i, _l, _l, _v, _l, _v, _l, _v]),
+    "atlg_dtrmm": (_i, [_i, _i, _i, _i, _l, _l, _d, _v, _l, _v, _l, _v]),
+    "atlg_dsymm": (_i, [_i, _i, _l, _l, _d, _v, _l, _v, _l, _d, _v, _l, _v]),
+    "atlg_dsyr2k": (_i, [_i, _i, _l, _l, _d, _v, _l, _v, _l, _d, _v, _l, _v]),
+    "atlg_dtrtri": (_i, [_i, _i, _l, _v, _l, _v, _v]),
+    "atlg_dlauum": (_i, [_i, _l, _v, _l, _v]),
+    "atlg_dpotri": (_i, [_i, _l, _v, _l, _v, _v]),
     "atlg_dpotrf_adjoint": (_i, [_l, _v, _l, _v, _l, _v]),
     "atlg_dtrsm_adjoint": (_i, [_l, _l, _v, _l, _v, _l, _v, _l, _v, _l, _v]),
     "atlg_dlogdet_chol": (_i, [_l, _v, _l, _v, _v]),
@@ -265,6 +302,70 @@ def ATL_dlaswp(n, a, lda, k1, k2, ipiv, inci):
     _host(lib().ATL_dlaswp, n, _p(a), lda, k1, k2, _pi(ipiv), inci)
 
 
+def cblas_dtrmm(order, side, uplo, ta, diag, m, n, alpha, a, lda, b, ldb):
+    _host(lib().cblas_dtrmm, order, side, uplo, ta, diag, m, n, alpha, _p(a), lda, _p(b), ldb)
+
+
+def cblas_dsymm(order, side, uplo, m, n, alpha, a, lda, b, ldb, beta, c, ldc):
+    _host(lib().cblas_dsymm, order, side, uplo, m, n, alpha, _p(a), lda, _p(b), ldb, beta, _p(c),
+          ldc)
+
+
+def cblas_dsyr2k(order, uplo, trans, n, k, alpha, a, lda, b, ldb, beta, c, ldc):
+    _host(lib().cblas_dsyr2k, order, uplo, trans, n, k, alpha, _p(a), lda, _p(b), ldb, beta,
+          _p(c), ldc)
+
+
+def clapack_dtrtri(order, uplo, diag, n, a, lda):
+    return _host(lib().clapack_dtrtri, order, uplo, diag, n, _p(a), lda)
+
+
+def clapack_dpotri(order, uplo, n, a, lda):
+    return _host(lib().clapack_dpotri, order, uplo, n, _p(a), lda)
+
+
+def clapack_dlauum(order, uplo, n, a, lda):
+    return _host(lib().clapack_dlauum, order, uplo, n, _p(a), lda)
+
+
+class F77:
+    """The Fortran 77 entry points (dgemm_, dgetrf_, ...) called the way a
+    Fortran program calls them: every scalar by reference, characters as
+    one-byte strings, 1-based pivots."""
+
+    @staticmethod
+    def _i(v):
+        return ctypes.byref(ctypes.c_int(int(v)))
+
+    @staticmethod
+    def _d(v):
+        return ctypes.byref(ctypes.c_double(float(v)))
+
+    @staticmethod
+    def _c(v):
+        return v.encode() if isinstance(v, str) else v
+
+    @classmethod
+    def call(cls, name, *args):
+        """args: ('c', 'N'), ('i', 5), ('d', 1.5), ('a', ndarray) or
+        ('info', None) (an int returned)."""
+        out, conv = None, []
+        for kind, v in args:
+            if kind == "c":
+                conv.append(cls._c(v))
+            elif kind == "i":
+                conv.append(cls._i(v))
+            elif kind == "d":
+                conv.append(cls._d(v))
+            elif kind == "a":
+                conv.append(_p(v) if v.dtype == np.float64 else _pi(v))
+            elif kind == "info":
+                out = ctypes.c_int(0)
+                conv.append(ctypes.byref(out))
+        _host(getattr(lib(), name), *conv)
+        return None if out is None else out.value
+
+
 def ATL_dgemm(ta, tb, m, n, k, alpha, a, lda, b, ldb, beta, c, ldc):
     _host(lib().ATL_dgemm, ta, tb, m, n, k, alpha, _p(a), lda, _p(b), ldb, beta, _p(c), ldc)
 
@@ -381,6 +482,34 @@ def dpotrf(uplo, n, a, lda, info, stream=None):
 def dpotrs(uplo, n, nrhs, a, lda, b, ldb, stream=None):
     _check(lib().atlg_dpotrs(uplo, n, nrhs, _ptr(a), lda, _ptr(b), ldb, _stream(stream)),
            "atlg_dpotrs")
+
+
+def dtrmm(side, uplo, ta, diag, m, n, alpha, a, lda, b, ldb, stream=None):
+    _check(lib().atlg_dtrmm(side, uplo, ta, diag, m, n, alpha, _ptr(a), lda, _ptr(b), ldb,
+                            _stream(stream)), "atlg_dtrmm")
+
+
+def dsymm(side, uplo, m, n, alpha, a, lda, b, ldb, beta, c, ldc, stream=None):
+    _check(lib().atlg_dsymm(side, uplo, m, n, alpha, _ptr(a), lda, _ptr(b), ldb, beta, _ptr(c),
+                            ldc, _stream(stream)), "atlg_dsymm")
+
+
+def dsyr2k(uplo, trans, n, k, alpha, a, lda, b, ldb, beta, c, ldc, stream=None):
+    _check(lib().atlg_dsyr2k(uplo, trans, n, k, alpha, _ptr(a), lda, _ptr(b), ldb, beta, _ptr(c),
+                             ldc, _stream(stream)), "atlg_dsyr2k")
+
+
+def dtrtri(uplo, diag, n, a, lda, info, stream=None):
+    _check(lib().atlg_dtrtri(uplo, diag, n, _ptr(a), lda, _ptr(info), _stream(stream)),
+           "atlg_dtrtri")
+
+
+def dlauum(uplo, n, a, lda, stream=None):
+    _check(lib().atlg_dlauum(uplo, n, _ptr(a), lda, _stream(stream)), "atlg_dlauum")
+
+
+def dpotri(uplo, n, a, lda, info, stream=None):
+    _check(lib().atlg_dpotri(uplo, n, _ptr(a), lda, _ptr(info), _stream(stream)), "atlg_dpotri")
 
 
 def dpotrf_adjoint(n, L, ldl, G, ldg, stream=None):

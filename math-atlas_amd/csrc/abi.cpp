@@ -200,6 +200,89 @@ int h_potrf(int uplo, long N, double* A, long lda) {
   return info;
 }
 
+void h_trmm(int Side, int Uplo, int TA, int Diag, long M, long N, double alpha, const double* A,
+            long lda, double* B, long ldb) {
+  if (M == 0 || N == 0) return;
+  hipStream_t s = lib_stream();
+  const long na = (Side == kLeft) ? M : N;
+  Stage a, b;
+  a.in(s, kSlotStageA, A, na, na, lda, true);
+  b.in(s, kSlotStageB, B, M, N, ldb, alpha != 0.0);
+  trmm(s, Side, Uplo, TA, Diag, M, N, alpha, a.d, lda, b.d, ldb);
+  b.out(s, B, M, N, ldb);
+  ATLG_HIP(hipStreamSynchronize(s));
+}
+
+void h_symm(int Side, int Uplo, long M, long N, double alpha, const double* A, long lda,
+            const double* B, long ldb, double beta, double* C, long ldc) {
+  if (M == 0 || N == 0) return;
+  hipStream_t s = lib_stream();
+  const long na = (Side == kLeft) ? M : N;
+  Stage a, b, c;
+  if (alpha != 0.0) {
+    a.in(s, kSlotStageA, A, na, na, lda, true);
+    b.in(s, kSlotStageB, B, M, N, ldb, true);
+  }
+  c.in(s, kSlotStageC, C, M, N, ldc, beta != 0.0);
+  symm(s, Side, Uplo, M, N, alpha, a.d, lda, b.d, ldb, beta, c.d, ldc);
+  c.out(s, C, M, N, ldc);
+  ATLG_HIP(hipStreamSynchronize(s));
+}
+
+void h_syr2k(int Uplo, int Trans, long N, long K, double alpha, const double* A, long lda,
+             const double* B, long ldb, double beta, double* C, long ldc) {
+  if (N == 0) return;
+  hipStream_t s = lib_stream();
+  const bool tr = is_trans(Trans);
+  Stage a, b, c;
+  if (K > 0 && alpha != 0.0) {
+    a.in(s, kSlotStageA, A, tr ? K : N, tr ? N : K, lda, true);
+    b.in(s, kSlotStageB, B, tr ? K : N, tr ? N : K, ldb, true);
+  }
+  c.in(s, kSlotStageC, C, N, N, ldc, true);  // the other triangle is kept
+  syr2k(s, Uplo, Trans, N, K, alpha, a.d, lda, b.d, ldb, beta, c.d, ldc);
+  c.out(s, C, N, N, ldc);
+  ATLG_HIP(hipStreamSynchronize(s));
+}
+
+int h_trtri(int uplo, int diag, long N, double* A, long lda) {
+  if (N == 0) return 0;
+  hipStream_t s = lib_stream();
+  Stage a;
+  a.in(s, kSlotStageA, A, N, N, lda, true);
+  int* dinfo = info_word(s);
+  trtri(s, uplo, diag, N, a.d, lda, dinfo);
+  a.out(s, A, N, N, lda);
+  int info = 0;
+  ATLG_HIP(hipMemcpyAsync(&info, dinfo, sizeof(int), hipMemcpyDeviceToHost, s));
+  ATLG_HIP(hipStreamSynchronize(s));
+  return info;
+}
+
+void h_lauum(int uplo, long N, double* A, long lda) {
+  if (N == 0) return;
+  hipStream_t s = lib_stream();
+  Stage a;
+  a.in(s, kSlotStageA, A, N, N, lda, true);
+  lauum(s, uplo, N, a.d, lda);
+  a.out(s, A, N, N, lda);
+  ATLG_HIP(hipStreamSynchronize(s));
+}
+
+int h_potri(int uplo, long N, double* A, long lda) {
+  if (N == 0) return 0;
+  hipStream_t s = lib_stream();
+  Stage a;
+  a.in(s, kSlotStageA, A, N, N, lda, true);
+  int* dinfo = info_word(s);
+  potri(s, uplo, N, a.d, lda, dinfo);
+  a.out(s, A, N, N, lda);
+  int info = 0;
+  ATLG_HIP(hipMemcpyAsync(&info, dinfo, sizeof(int), hipMemcpyDeviceToHost, s));
+  ATLG_HIP(hipStreamSynchronize(s));
+  return info;
+}
+
 // ATL_dpotrs, both orders (atlas-lp.base:969-1050); RowMajor B as in h_getrs.
 void h_potrs(int Order, int Uplo, long N, long NRHS, const double* A, long lda, double* B,
              long ldb) {
@@ -281,6 +364,20 @@ __attribute__((weak)) void cblas_xerbla(int p, const char* rout, const char* for
 
 void atlg_set_xerbla_hook(void (*hook)(int, const char*, const char*)) { g_xerbla_hook = hook; }
 
+}  // extern "C"
+
+void atlg::f77_xerbla(int p, const char* rout, const char* msg) {
+  if (g_xerbla_hook) {
+    g_xerbla_hook(p, rout, msg);
+    return;
+  }
+  fputs(msg, stdout);
+  fflush(stdout);
+  exit(0);
+}
+
+extern "C" {
+
 // ---- CBLAS -----------------------------------------------------------------
 
 // cblas_dgemm (AtlasBase/Clint/cblas.base:110-236)
@@ -360,11 +457,13 @@ void cblas_dgemm(const enum CBLAS_ORDER Order, const enum CBLAS_TRANSPOSE TA,
   });
 }
 
-// cblas_dtrsm (AtlasBase/Clint/cblas.base:587-693)
-void cblas_dtrsm(const enum CBLAS_ORDER Order, const enum CBLAS_SIDE Side,
-                 const enum CBLAS_UPLO Uplo, const enum CBLAS_TRANSPOSE TA,
-                 const enum CBLAS_DIAG Diag, const int M, const int N, const double alpha,
-                 const double* A, const int lda, double* B, const int ldb) {
+}  // extern "C"
+
+namespace {
+// The argument checks cblas_dtrsm and cblas_dtrmm share (one wrapper in
+// AtlasBase/Clint/cblas.base:587-693); 2000 when every argument is legal.
+int trxm_checks(int Order, int Side, int Uplo, int TA, int Diag, int M, int N, int lda,
+                int ldb) {
   int info = 2000;
   if (Order == CblasColMajor) {
     if (Side == CblasLeft) {
@@ -407,6 +506,18 @@ void cblas_dtrsm(const enum CBLAS_ORDER Order, const enum CBLAS_SIDE Side,
                   Diag);
   if (M < 0) info = errprn(6, info, "M cannot be less than zero; it is set to %d.", M);
   if (N < 0) info = errprn(7, info, "N cannot be less than zero; it is set to %d.", N);
+  return info;
+}
+}  // namespace
+
+extern "C" {
+
+// cblas_dtrsm (AtlasBase/Clint/cblas.base:587-693)
+void cblas_dtrsm(const enum CBLAS_ORDER Order, const enum CBLAS_SIDE Side,
+                 const enum CBLAS_UPLO Uplo, const enum CBLAS_TRANSPOSE TA,
+                 const enum CBLAS_DIAG Diag, const int M, const int N, const double alpha,
+                 const double* A, const int lda, double* B, const int ldb) {
+  const int info = trxm_checks(Order, Side, Uplo, TA, Diag, M, N, lda, ldb);
   if (info != 2000) {
     cblas_xerbla(info, "cblas_dtrsm", "");
     return;
@@ -417,6 +528,120 @@ void cblas_dtrsm(const enum CBLAS_ORDER Order, const enum CBLAS_SIDE Side,
     else
       h_trsm(Side == CblasLeft ? kRight : kLeft, Uplo == CblasUpper ? kLower : kUpper, TA, Diag,
              N, M, alpha, A, lda, B, ldb);
+  });
+}
+
+// cblas_dtrmm (AtlasBase/Clint/cblas.base:587-693, the TRSM/TRMM wrapper)
+void cblas_dtrmm(const enum CBLAS_ORDER Order, const enum CBLAS_SIDE Side,
+                 const enum CBLAS_UPLO Uplo, const enum CBLAS_TRANSPOSE TA,
+                 const enum CBLAS_DIAG Diag, const int M, const int N, const double alpha,
+                 const double* A, const int lda, double* B, const int ldb) {
+  const int info = trxm_checks(Order, Side, Uplo, TA, Diag, M, N, lda, ldb);
+  if (info != 2000) {
+    cblas_xerbla(info, "cblas_dtrmm", "");
+    return;
+  }
+  locked("cblas_dtrmm", [&] {
+    if (Order == CblasColMajor)
+      h_trmm(Side, Uplo, TA, Diag, M, N, alpha, A, lda, B, ldb);
+    else
+      h_trmm(Side == CblasLeft ? kRight : kLeft, Uplo == CblasUpper ? kLower : kUpper, TA, Diag,
+             N, M, alpha, A, lda, B, ldb);
+  });
+}
+
+// cblas_dsymm (AtlasBase/Clint/cblas.base:237-338)
+void cblas_dsymm(const enum CBLAS_ORDER Order, const enum CBLAS_SIDE Side,
+                 const enum CBLAS_UPLO Uplo, const int M, const int N, const double alpha,
+                 const double* A, const int lda, const double* B, const int ldb,
+                 const double beta, double* C, const int ldc) {
+  int info = 2000;
+  if (Order == CblasColMajor || Order == CblasRowMajor) {
+    const bool cm = Order == CblasColMajor;
+    if (Side == CblasLeft) {
+      if (lda < M || lda < 1)
+        info = errprn(8, info, "lda must be >= MAX(M,1): lda=%d M=%d", lda, M);
+    } else if (Side == CblasRight) {
+      if (lda < N || lda < 1)
+        info = errprn(8, info, "lda must be >= MAX(N,1): lda=%d N=%d", lda, N);
+    } else {
+      info = errprn(2, info, "SIDE must be %d or %d, but is set to %d", CblasRight, CblasLeft,
+                    Side);
+    }
+    if (cm) {
+      if (ldb < M || ldb < 1)
+        info = errprn(10, info, "ldb must be >= MAX(M,1): ldb=%d M=%d", ldb, M);
+      if (ldc < M || ldc < 1)
+        info = errprn(13, info, "ldc must be >= MAX(M,1): ldc=%d M=%d", ldc, M);
+    } else {
+      if (ldb < N || ldb < 1)
+        info = errprn(10, info, "ldb must be >= MAX(N,1): ldb=%d N=%d", ldb, N);
+      if (ldc < N || ldc < 1)
+        info = errprn(13, info, "ldc must be >= MAX(N,1): ldc=%d N=%d", ldc, N);
+    }
+  } else {
+    info = errprn(1, info, "Order must be %d or %d, but is set to %d", CblasRowMajor,
+                  CblasColMajor, Order);
+  }
+  if (Uplo != CblasUpper && Uplo != CblasLower)
+    info = errprn(3, info, "UPLO must be %d or %d, but is set to %d", CblasUpper, CblasLower,
+                  Uplo);
+  if (M < 0) info = errprn(4, info, "M cannot be less than zero; it is set to %d.", M);
+  if (N < 0) info = errprn(5, info, "N cannot be less than zero; it is set to %d.", N);
+  if (info != 2000) {
+    cblas_xerbla(info, "cblas_dsymm", "");
+    return;
+  }
+  locked("cblas_dsymm", [&] {
+    if (Order == CblasColMajor)
+      h_symm(Side, Uplo, M, N, alpha, A, lda, B, ldb, beta, C, ldc);
+    else
+      h_symm(Side == CblasLeft ? kRight : kLeft, Uplo == CblasUpper ? kLower : kUpper, N, M,
+             alpha, A, lda, B, ldb, beta, C, ldc);
+  });
+}
+
+// cblas_dsyr2k (AtlasBase/Clint/cblas.base:450-586)
+void cblas_dsyr2k(const enum CBLAS_ORDER Order, const enum CBLAS_UPLO Uplo,
+                  const enum CBLAS_TRANSPOSE Trans, const int N, const int K, const double alpha,
+                  const double* A, const int lda, const double* B, const int ldb,
+                  const double beta, double* C, const int ldc) {
+  int info = 2000;
+  if (Uplo != CblasUpper && Uplo != CblasLower)
+    info = errprn(2, info, "UPLO must be %d or %d, but is set to %d", CblasUpper, CblasLower,
+                  Uplo);
+  if (N < 0) info = errprn(4, info, "N cannot be less than zero; it is set to %d.", N);
+  if (K < 0) info = errprn(5, info, "K cannot be less than zero; it is set to %d.", K);
+  if (Order == CblasColMajor || Order == CblasRowMajor) {
+    // the long dimension of A and B: N for (ColMajor, NoTrans) and
+    // (RowMajor, Trans), K otherwise
+    const bool nt = Trans == CblasNoTrans;
+    if (!nt && Trans != CblasTrans && Trans != CblasConjTrans)
+      info = errprn(3, info, "Trans must be %d or %d, but is set to %d", CblasNoTrans, CblasTrans,
+                    Trans);
+    const bool byN = (Order == CblasColMajor) == nt;
+    const int need = byN ? N : K;
+    const char* nm = byN ? "N" : "K";
+    if (lda < need || lda < 1)
+      info = errprn(8, info, "lda must be >= MAX(%s,1): lda=%d %s=%d", nm, lda, nm, need);
+    if (ldb < need || ldb < 1)
+      info = errprn(10, info, "ldb must be >= MAX(%s,1): ldb=%d %s=%d", nm, ldb, nm, need);
+  } else {
+    info = errprn(1, info, "Order must be %d or %d, but is set to %d", CblasRowMajor,
+                  CblasColMajor, Order);
+  }
+  if (ldc < N || ldc < 1)
+    info = errprn(13, info, "ldc must be >= MAX(N,1): ldc=%d N=%d", ldc, N);
+  if (info != 2000) {
+    cblas_xerbla(info, "cblas_dsyr2k", "");
+    return;
+  }
+  locked("cblas_dsyr2k", [&] {
+    if (Order == CblasColMajor)
+      h_syr2k(Uplo, Trans, N, K, alpha, A, lda, B, ldb, beta, C, ldc);
+    else
+      h_syr2k(Uplo == CblasUpper ? kLower : kUpper, Trans == CblasNoTrans ? kTrans : kNoTrans, N,
+              K, alpha, A, lda, B, ldb, beta, C, ldc);
   });
 }
 
@@ -547,7 +772,133 @@ void ATL_dpotrs(const enum CBLAS_ORDER Order, const enum CBLAS_UPLO Uplo, const 
   locked("ATL_dpotrs", [&] { h_potrs(Order, Uplo, N, NRHS, A, lda, B, ldb); });
 }
 
+void ATL_dtrmm(const enum CBLAS_SIDE Side, const enum CBLAS_UPLO Uplo,
+               const enum CBLAS_TRANSPOSE TA, const enum CBLAS_DIAG Diag, ATL_CSZT M, ATL_CSZT N,
+               const double alpha, const double* A, ATL_CSZT lda, double* B, ATL_CSZT ldb) {
+  locked("ATL_dtrmm", [&] {
+    h_trmm(Side, Uplo, TA, Diag, (long)M, (long)N, alpha, A, (long)lda, B, (long)ldb);
+  });
+}
+
+void ATL_dsymm(const enum CBLAS_SIDE Side, const enum CBLAS_UPLO Uplo, ATL_CSZT M, ATL_CSZT N,
+               const double alpha, const double* A, ATL_CSZT lda, const double* B, ATL_CSZT ldb,
+               const double beta, double* C, ATL_CSZT ldc) {
+  locked("ATL_dsymm", [&] {
+    h_symm(Side, Uplo, (long)M, (long)N, alpha, A, (long)lda, B, (long)ldb, beta, C, (long)ldc);
+  });
+}
+
+void ATL_dsyr2k(const enum CBLAS_UPLO Uplo, const enum CBLAS_TRANSPOSE Trans, ATL_CSZT N,
+                ATL_CSZT K, const double alpha, const double* A, ATL_CSZT lda, const double* B,
+                ATL_CSZT ldb, const double beta, double* C, ATL_CSZT ldc) {
+  locked("ATL_dsyr2k", [&] {
+    h_syr2k(Uplo, Trans, (long)N, (long)K, alpha, A, (long)lda, B, (long)ldb, beta, C,
+            (long)ldc);
+  });
+}
+
+// ATL_trtri (atlas-lp.base:2341-2380): a zero on a NonUnit diagonal returns
+// its 1-based index and leaves A unchanged.  Row-major is the column-major
+// inverse of the transpose, i.e. the other triangle of the same memory.
+int ATL_dtrtri(const enum CBLAS_ORDER Order, const enum CBLAS_UPLO Uplo,
+               const enum CBLAS_DIAG Diag, const int N, double* A, const int lda) {
+  const bool lower = (Order == CblasColMajor) == (Uplo == CblasLower);
+  return locked("ATL_dtrtri", [&] { return h_trtri(lower ? kLower : kUpper, Diag, N, A, lda); });
+}
+
+// ATL_lauum (atlas-lp.base:312-330): lower A <- L^T L, upper A <- U U^T;
+// row-major is the column-major routine for the other triangle.
+void ATL_dlauum(const enum CBLAS_ORDER Order, const enum CBLAS_UPLO Uplo, const int N,
+                double* A, const int lda) {
+  const bool lower = (Order == CblasColMajor) == (Uplo == CblasLower);
+  locked("ATL_dlauum", [&] { h_lauum(lower ? kLower : kUpper, N, A, lda); });
+}
+
+// The SPD inverse from the Cholesky factor (clapack_dpotri's trtri + lauum,
+// atlas-clp.base:316-362).
+int ATL_dpotri(const enum CBLAS_ORDER Order, const enum CBLAS_UPLO Uplo, const int N, double* A,
+               const int lda) {
+  const bool lower = (Order == CblasColMajor) == (Uplo == CblasLower);
+  return locked("ATL_dpotri", [&] { return h_potri(lower ? kLower : kUpper, N, A, lda); });
+}
+
 // ---- CLAPACK (AtlasBase/Clint/atlas-clp.base) ------------------------------
+
+// clapack_dtrtri (atlas-clp.base:411-452).  The reference reads its error
+// code uninitialized and calls ATL_trtri only when it is nonzero; here the
+// inverse is computed exactly when every argument is legal.
+int clapack_dtrtri(const enum CBLAS_ORDER Order, const enum CBLAS_UPLO Uplo,
+                   const enum CBLAS_DIAG Diag, const int N, double* A, const int lda) {
+  int ierr = 0;
+  if (Order != CblasRowMajor && Order != CblasColMajor) {
+    ierr = -1;
+    cblas_xerbla(1, "clapack_dtrtri", "Order must be %d or %d, but is set to %d\n",
+                 CblasRowMajor, CblasColMajor, Order);
+  }
+  if (Uplo != CblasUpper && Uplo != CblasLower) {
+    ierr = -2;
+    cblas_xerbla(2, "clapack_dtrtri", "Uplo must be %d or %d, but is set to %d\n", CblasUpper,
+                 CblasLower, Uplo);
+  }
+  if (Diag != CblasUnit && Diag != CblasNonUnit) {
+    ierr = -3;
+    cblas_xerbla(3, "clapack_dtrtri", "Diag must be %d or %d, but is set to %d\n", CblasNonUnit,
+                 CblasUnit, Diag);
+  }
+  if (N < 0) {
+    ierr = -4;
+    cblas_xerbla(4, "clapack_dtrtri", "N cannot be less than zero 0,; is set to %d.\n", N);
+  }
+  if (lda < N || lda < 1) {
+    ierr = -6;
+    cblas_xerbla(6, "clapack_dtrtri", "lda must be >= MAX(N,1): lda=%d N=%d\n", lda, N);
+  }
+  if (!ierr) ierr = ATL_dtrtri(Order, Uplo, Diag, N, A, lda);
+  return ierr;
+}
+
+namespace {
+// The checks clapack_dpotri and clapack_dlauum share with clapack_dpotrf
+// (atlas-clp.base:316-356); 0 or the negative error code.
+int potri_like_checks(const char* rt, int Order, int Uplo, int N, int lda) {
+  int ierr = 0;
+  if (Order != CblasRowMajor && Order != CblasColMajor) {
+    ierr = -1;
+    cblas_xerbla(1, rt, "Order must be %d or %d, but is set to %d\n", CblasRowMajor,
+                 CblasColMajor, Order);
+  }
+  if (Uplo != CblasUpper && Uplo != CblasLower) {
+    ierr = -2;
+    cblas_xerbla(2, rt, "Uplo must be %d or %d, but is set to %d\n", CblasUpper, CblasLower,
+                 Uplo);
+  }
+  if (N < 0) {
+    ierr = -3;
+    cblas_xerbla(3, rt, "N cannot be less than zero 0,; is set to %d.\n", N);
+  }
+  if (lda < N || lda < 1) {
+    ierr = -5;
+    cblas_xerbla(5, rt, "lda must be >= MAX(N,1): lda=%d N=%d\n", lda, N);
+  }
+  return ierr;
+}
+}  // namespace
+
+// clapack_dpotri (atlas-clp.base:316-362): trtri, then lauum if it succeeded.
+int clapack_dpotri(const enum CBLAS_ORDER Order, const enum CBLAS_UPLO Uplo, const int N,
+                   double* A, const int lda) {
+  int ierr = potri_like_checks("clapack_dpotri", Order, Uplo, N, lda);
+  if (!ierr) ierr = ATL_dpotri(Order, Uplo, N, A, lda);
+  return ierr;
+}
+
+// clapack_dlauum (atlas-clp.base:316-362)
+int clapack_dlauum(const enum CBLAS_ORDER Order, const enum CBLAS_UPLO Uplo, const int N,
+                   double* A, const int lda) {
+  const int ierr = potri_like_checks("clapack_dlauum", Order, Uplo, N, lda);
+  if (!ierr) ATL_dlauum(Order, Uplo, N, A, lda);
+  return ierr;
+}
 
 int clapack_dgetrf(const enum CBLAS_ORDER Order, const int M, const int N, double* A,
                    const int lda, int* ipiv) {

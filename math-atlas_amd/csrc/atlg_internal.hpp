@@ -24,6 +24,11 @@ inline bool is_trans(int t) { return t == kTrans || t == kConjTrans; }
 // Records a failure message for atlg_last_error (thread-local).
 void set_error(const std::string& msg);
 
+// The Fortran XERBLA's report: the hook of atlg_set_xerbla_hook when one is
+// installed (p = INFO), else the message on stdout and exit(0), as the
+// reference XERBLA's WRITE and STOP do.
+void f77_xerbla(int p, const char* rout, const char* msg);
+
 // Thrown inside drivers on a HIP failure; entry points convert it to
 // ATLG_EHIP.
 struct HipFailure {
@@ -91,7 +96,9 @@ enum Slot : int {
   kSlotLaLists = 16,    // per-leaf interchange lists of the look-ahead LU
   kSlotLaPerm = 17,     // row maps of the look-ahead LU's deferred interchanges
   kSlotLaTemp = 18,     // staging of one column block for those interchanges
-  kNumSlots = 19
+  kSlotTrmmBlk = 19,    // dense diagonal blocks of op(A) for TRMM leaves
+  kSlotSymm = 20,       // dense symmetric copy of A for SYMM
+  kNumSlots = 21
 };
 void* scratch(hipStream_t s, Slot slot, size_t bytes);
 
@@ -161,6 +168,26 @@ void* noalloc_scratch();
 
 template <class S>
 void laswp(hipStream_t s, long N, S* A, long lda, int K1, int K2, const int* ipiv, int inci);
+
+// The rest of the real level-3 BLAS and the triangular / SPD inverses
+// (blas3x.hip).  trmm: B <- alpha op(A) B (Left) or alpha B op(A) (Right);
+// symm: C <- alpha A B + beta C (Left) or alpha B A + beta C (Right), A
+// symmetric with only `uplo` stored; syr2k: C_uplo <- alpha (A B^T + B A^T)
+// + beta C_uplo (NoTrans) or alpha (A^T B + B^T A) + beta C_uplo (Trans);
+// trtri: A <- inv(A) in place (triangle only), *dinfo = first zero
+// diagonal (1-based, NonUnit) and then A is untouched; lauum: lower
+// A <- L^T L, upper A <- U U^T (triangle only); potri: the SPD inverse
+// from the Cholesky factor, trtri then lauum.  `skip` (device word, may
+// be null): nonzero makes every launch a no-op.
+void trmm(hipStream_t s, int side, int uplo, int ta, int diag, long M, long N, double alpha,
+          const double* A, long lda, double* B, long ldb, const int* skip = nullptr);
+void symm(hipStream_t s, int side, int uplo, long M, long N, double alpha, const double* A,
+          long lda, const double* B, long ldb, double beta, double* C, long ldc);
+void syr2k(hipStream_t s, int uplo, int trans, long N, long K, double alpha, const double* A,
+           long lda, const double* B, long ldb, double beta, double* C, long ldc);
+void trtri(hipStream_t s, int uplo, int diag, long N, double* A, long lda, int* dinfo);
+void lauum(hipStream_t s, int uplo, long N, double* A, long lda, const int* skip = nullptr);
+void potri(hipStream_t s, int uplo, long N, double* A, long lda, int* dinfo);
 
 // Precision of getrf's trailing-update GEMMs: the scalar type's own MFMA,
 // or (float only) BF16 operands with FP32 accumulation, single or split

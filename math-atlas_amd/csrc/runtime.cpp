@@ -402,6 +402,76 @@ int atlg_dpotrs(int Uplo, long N, long NRHS, const double* A, long lda,
   return guarded([&] { potrs(resolve_stream(stream), Uplo, N, NRHS, A, lda, B, ldb); });
 }
 
+int atlg_dtrmm(int Side, int Uplo, int TA, int Diag, long M, long N, double alpha,
+               const double* A, long lda, double* B, long ldb, atlg_stream_t stream) {
+  if (bad_side(Side)) return -1;
+  if (bad_uplo(Uplo)) return -2;
+  if (bad_trans(TA)) return -3;
+  if (bad_diag(Diag)) return -4;
+  if (M < 0) return -5;
+  if (N < 0) return -6;
+  if (lda < std::max(1L, Side == kLeft ? M : N)) return -9;
+  if (ldb < std::max(1L, M)) return -11;
+  return guarded([&] {
+    trmm(resolve_stream(stream), Side, Uplo, TA, Diag, M, N, alpha, A, lda, B, ldb);
+  });
+}
+
+int atlg_dsymm(int Side, int Uplo, long M, long N, double alpha, const double* A, long lda,
+               const double* B, long ldb, double beta, double* C, long ldc,
+               atlg_stream_t stream) {
+  if (bad_side(Side)) return -1;
+  if (bad_uplo(Uplo)) return -2;
+  if (M < 0) return -3;
+  if (N < 0) return -4;
+  if (lda < std::max(1L, Side == kLeft ? M : N)) return -7;
+  if (ldb < std::max(1L, M)) return -9;
+  if (ldc < std::max(1L, M)) return -12;
+  return guarded([&] {
+    symm(resolve_stream(stream), Side, Uplo, M, N, alpha, A, lda, B, ldb, beta, C, ldc);
+  });
+}
+
+int atlg_dsyr2k(int Uplo, int Trans, long N, long K, double alpha, const double* A, long lda,
+                const double* B, long ldb, double beta, double* C, long ldc,
+                atlg_stream_t stream) {
+  if (bad_uplo(Uplo)) return -1;
+  if (bad_trans(Trans)) return -2;
+  if (N < 0) return -3;
+  if (K < 0) return -4;
+  if (lda < std::max(1L, is_trans(Trans) ? K : N)) return -7;
+  if (ldb < std::max(1L, is_trans(Trans) ? K : N)) return -9;
+  if (ldc < std::max(1L, N)) return -12;
+  return guarded([&] {
+    syr2k(resolve_stream(stream), Uplo, Trans, N, K, alpha, A, lda, B, ldb, beta, C, ldc);
+  });
+}
+
+int atlg_dtrtri(int Uplo, int Diag, long N, double* A, long lda, int* dinfo,
+                atlg_stream_t stream) {
+  if (bad_uplo(Uplo)) return -1;
+  if (bad_diag(Diag)) return -2;
+  if (N < 0) return -3;
+  if (lda < std::max(1L, N)) return -5;
+  if (!dinfo) return -6;
+  return guarded([&] { trtri(resolve_stream(stream), Uplo, Diag, N, A, lda, dinfo); });
+}
+
+int atlg_dlauum(int Uplo, long N, double* A, long lda, atlg_stream_t stream) {
+  if (bad_uplo(Uplo)) return -1;
+  if (N < 0) return -2;
+  if (lda < std::max(1L, N)) return -4;
+  return guarded([&] { lauum(resolve_stream(stream), Uplo, N, A, lda); });
+}
+
+int atlg_dpotri(int Uplo, long N, double* A, long lda, int* dinfo, atlg_stream_t stream) {
+  if (bad_uplo(Uplo)) return -1;
+  if (N < 0) return -2;
+  if (lda < std::max(1L, N)) return -4;
+  if (!dinfo) return -5;
+  return guarded([&] { potri(resolve_stream(stream), Uplo, N, A, lda, dinfo); });
+}
+
 int atlg_dpotrf_adjoint(long N, const double* L, long ldl, double* G, long ldg,
                         atlg_stream_t stream) {
   if (N < 0) return -1;

@@ -234,16 +234,17 @@ __device__ inline int ld_flag(const int* p) {
 __device__ inline void st_flag(int* p, int v) {
   __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
-// Publication of a solved block: the flag store is a release at agent scope,
-// and the reader's flag load an acquire, so the x values stored before the
-// flag (by every thread, ordered by the agent-scope fence and the barrier
-// in front of the flag store) are visible once the flag reads 1.
-__device__ inline void st_flag_release(int* p, int v) {
-  __hip_atomic_store(p, v, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
-}
-__device__ inline int ld_flag_acquire(const int* p) {
-  return __hip_atomic_load(const_cast<int*>(p), __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT);
-}
+// Publication of a solved block.  Everything that crosses workgroups (the
+// solved x values and the flags) is accessed only with agent-scope atomic
+// loads and stores, which gfx950 performs coherently at the device level
+// (the sc1 form: stores write through, loads do not take a stale per-XCD L2
+// copy).  What the protocol needs is order: every thread waits until its
+// own x stores are acknowledged (s_waitcnt 0) before the workgroup barrier,
+// the flag is stored after the barrier, and a reader issues its x loads
+// only after its flag load has returned 1 (and a barrier).  Agent-scope
+// release/acquire fences would give the same order but also write back and
+// invalidate the whole L2 of the XCD at every block (buffer_wbl2 /
+// buffer_inv), which made each solve several times slower.
 
 __global__ void sv_reset(int* flags, int n) {
   for (int e = blockIdx.x * blockDim.x + threadIdx.x; e < n; e += gridDim.x * blockDim.x)
@@ -283,7 +284,7 @@ trsv_sf_kernel(int n, int nb, int upper, S alpha, const S* __restrict__ A, long 
         // one round trip per poll; the error word only every 256 polls
         bool good = true;
         long spins = 0;
-        while (ld_flag_acquire(&flags[q]) == 0) {
+        while (ld_flag(&flags[q]) == 0) {
           if (++spins >= limit || ((spins & 255) == 0 && ld_flag(err))) {
             st_flag(err, 1);
             atomicOr(status, ATLG_FAIL_TRSV);
@@ -293,11 +294,8 @@ trsv_sf_kernel(int n, int nb, int upper, S alpha, const S* __restrict__ A, long 
         }
         ok = good;
       }
-      __syncthreads();
+      __syncthreads();  // x_q is read (ld_co) only after the flag was seen
       if (!ok) return;  // same value in every thread
-      // thread 0's acquire covers its own later loads; the other threads
-      // read x after the barrier, so they take the acquire side as well
-      __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
       if (t < NB) {
 #pragma unroll
         for (int k = 0; k < R; k++) xs[t][k] = (t < cols) ? ld_co(&B[c0 + t + (size_t)k * ldb]) : S(0);
@@ -341,12 +339,11 @@ trsv_sf_kernel(int n, int nb, int upper, S alpha, const S* __restrict__ A, long 
 #pragma unroll
       for (int k = 0; k < R; k++) st_co(&B[r0 + t + (size_t)k * ldb], part[0][t][k] + part[1][t][k]);
     }
-    // every thread's stores of x reach the coherence point (agent-scope
-    // release: L2 writeback and a wait for the stores) before the barrier,
-    // and the flag is stored after it with release semantics
-    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+    // every thread's x stores are acknowledged before the barrier, and the
+    // flag is stored after it
+    __builtin_amdgcn_s_waitcnt(0);
     __syncthreads();
-    if (t == 0) st_flag_release(&flags[p], 1);
+    if (t == 0) st_flag(&flags[p], 1);
   }
 }
 

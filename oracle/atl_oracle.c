@@ -18,6 +18,7 @@
 #include <stdlib.h>
 #include <string.h>
 #include <time.h>
+#include <pthread.h>
 
 enum { NoTrans = 111, Trans = 112, ConjTrans = 113, Upper = 121, Lower = 122,
        NonUnit = 131, Unit = 132, Left = 141, Right = 142 };
@@ -796,6 +797,324 @@ int orc_npvts(int N, const int *ipiv)
    int i, n = 0;
    for (i = 0; i < N; i++) if (ipiv[i] != i) n++;
    return n;
+}
+
+/* ------------------------------------------------------------------------
+ * Textbook TRMM, SYMM, SYR2K (the ATL_ref* reference routines,
+ * AtlasBase/Antoine/atlrefblas.base: trmm :8406+, symm :7223+, syr2k
+ * :7576+) and the triangular inverse / LAUUM (LAPACK's definitions, as
+ * ATL_trtri and ATL_lauum compute them, AtlasBase/Clint/atlas-lp.base:
+ * 1703-2380, 221-330), by explicit sums over the referenced elements.
+ * ------------------------------------------------------------------------ */
+
+/* op(A)(i, k) of a triangular A with the given uplo / trans / diag */
+static double tri_elem(int uplo, int ta, int diag, const double *A, int lda, int i, int k)
+{
+   const int trans = ta != NoTrans;
+   const int lower_eff = (uplo == Lower) != trans;
+   if (lower_eff ? i < k : i > k) return 0.0;
+   if (i == k && diag == Unit) return 1.0;
+   return trans ? A[k + (size_t)i * lda] : A[i + (size_t)k * lda];
+}
+
+void orc_reftrmm(int side, int uplo, int ta, int diag, int M, int N, double alpha,
+                 const double *A, int lda, double *B, int ldb)
+{
+   double *T = (double *)malloc(sizeof(double) * (size_t)(M > 0 ? M : 1) * (N > 0 ? N : 1));
+   int i, j, k;
+   for (j = 0; j < N; j++)
+      for (i = 0; i < M; i++) T[i + (size_t)j * M] = B[i + (size_t)j * ldb];
+   for (j = 0; j < N; j++)
+      for (i = 0; i < M; i++)
+      {
+         double t = 0.0;
+         if (side == Left)
+            for (k = 0; k < M; k++) t += tri_elem(uplo, ta, diag, A, lda, i, k) * T[k + (size_t)j * M];
+         else
+            for (k = 0; k < N; k++) t += T[i + (size_t)k * M] * tri_elem(uplo, ta, diag, A, lda, k, j);
+         B[i + (size_t)j * ldb] = alpha * t;
+      }
+   free(T);
+}
+
+void orc_refsymm(int side, int uplo, int M, int N, double alpha, const double *A, int lda,
+                 const double *B, int ldb, double beta, double *C, int ldc)
+{
+   int i, j, k;
+   for (j = 0; j < N; j++)
+      for (i = 0; i < M; i++)
+      {
+         double t = 0.0, c;
+         if (side == Left)
+            for (k = 0; k < M; k++)
+            {
+               const int st = uplo == Lower ? i >= k : i <= k;
+               const double a = st ? A[i + (size_t)k * lda] : A[k + (size_t)i * lda];
+               t += a * B[k + (size_t)j * ldb];
+            }
+         else
+            for (k = 0; k < N; k++)
+            {
+               const int st = uplo == Lower ? k >= j : k <= j;
+               const double a = st ? A[k + (size_t)j * lda] : A[j + (size_t)k * lda];
+               t += B[i + (size_t)k * ldb] * a;
+            }
+         c = beta == 0.0 ? 0.0 : beta * C[i + (size_t)j * ldc];
+         C[i + (size_t)j * ldc] = alpha * t + c;
+      }
+}
+
+void orc_refsyr2k(int uplo, int trans, int N, int K, double alpha, const double *A, int lda,
+                  const double *B, int ldb, double beta, double *C, int ldc)
+{
+   const int tr = trans != NoTrans;
+   int i, j, k;
+   for (j = 0; j < N; j++)
+      for (i = 0; i < N; i++)
+      {
+         double t = 0.0, c;
+         if (uplo == Lower ? i < j : i > j) continue;
+         for (k = 0; k < K; k++)
+         {
+            const double ai = tr ? A[k + (size_t)i * lda] : A[i + (size_t)k * lda];
+            const double aj = tr ? A[k + (size_t)j * lda] : A[j + (size_t)k * lda];
+            const double bi = tr ? B[k + (size_t)i * ldb] : B[i + (size_t)k * ldb];
+            const double bj = tr ? B[k + (size_t)j * ldb] : B[j + (size_t)k * ldb];
+            t += ai * bj + bi * aj;
+         }
+         c = beta == 0.0 ? 0.0 : beta * C[i + (size_t)j * ldc];
+         C[i + (size_t)j * ldc] = alpha * t + c;
+      }
+}
+
+/* inv(T) of the uplo triangle by substitution, column by column (T X = I);
+ * a zero on a NonUnit diagonal returns its 1-based index, A unchanged. */
+int orc_reftrtri(int uplo, int diag, int N, double *A, int lda)
+{
+   double *X;
+   int i, j, k;
+   if (diag == NonUnit)
+      for (i = 0; i < N; i++)
+         if (A[i + (size_t)i * lda] == 0.0) return i + 1;
+   X = (double *)malloc(sizeof(double) * (size_t)(N > 0 ? N : 1) * (N > 0 ? N : 1));
+   for (j = 0; j < N; j++)
+   {
+      double *x = X + (size_t)j * N;
+      for (i = 0; i < N; i++) x[i] = (i == j) ? 1.0 : 0.0;
+      if (uplo == Lower)
+         for (i = 0; i < N; i++)
+         {
+            double t = x[i];
+            for (k = 0; k < i; k++) t -= A[i + (size_t)k * lda] * x[k];
+            x[i] = diag == Unit ? t : t / A[i + (size_t)i * lda];
+         }
+      else
+         for (i = N - 1; i >= 0; i--)
+         {
+            double t = x[i];
+            for (k = i + 1; k < N; k++) t -= A[i + (size_t)k * lda] * x[k];
+            x[i] = diag == Unit ? t : t / A[i + (size_t)i * lda];
+         }
+   }
+   for (j = 0; j < N; j++)
+      for (i = 0; i < N; i++)
+      {
+         if (uplo == Lower ? i < j : i > j) continue;
+         if (i == j && diag == Unit) continue;
+         A[i + (size_t)j * lda] = X[i + (size_t)j * N];
+      }
+   free(X);
+   return 0;
+}
+
+/* Lower: A <- L^T L; Upper: A <- U U^T (the uplo triangle). */
+void orc_reflauum(int uplo, int N, double *A, int lda)
+{
+   double *P = (double *)malloc(sizeof(double) * (size_t)(N > 0 ? N : 1) * (N > 0 ? N : 1));
+   int i, j, k;
+   for (j = 0; j < N; j++)
+      for (i = 0; i < N; i++)
+      {
+         double t = 0.0;
+         if (uplo == Lower ? i < j : i > j) continue;
+         if (uplo == Lower)
+            for (k = i; k < N; k++) t += A[k + (size_t)i * lda] * A[k + (size_t)j * lda];
+         else
+            for (k = j; k < N; k++) t += A[i + (size_t)k * lda] * A[j + (size_t)k * lda];
+         P[i + (size_t)j * N] = t;
+      }
+   for (j = 0; j < N; j++)
+      for (i = 0; i < N; i++)
+         if (uplo == Lower ? i >= j : i <= j) A[i + (size_t)j * lda] = P[i + (size_t)j * N];
+   free(P);
+}
+
+/* ------------------------------------------------------------------------
+ * The netlib CBLAS level-3 tester's machinery, restated in C (its Fortran
+ * driver c_dblat3.f needs a Fortran compiler): the DBEG generator, DMAKE and
+ * the DMMCH check (AtlasBase/Clint/misc/CBLATST/c_dblat3.f).
+ * ------------------------------------------------------------------------ */
+
+/* DBEG (c_dblat3.f, FUNCTION DBEG): values (i - 500)/1001 from the
+ * multiplicative sequence i <- 891 i mod 1000, every fifth step skipped;
+ * st[0..2] = (i, ic, mi), st[3] != 0 asks for a reset. */
+double orc_blat_dbeg(int *st)
+{
+   if (st[3])
+   {
+      st[2] = 891;
+      st[0] = 7;
+      st[1] = 0;
+      st[3] = 0;
+   }
+   st[1]++;
+   for (;;)
+   {
+      st[0] = st[0] * st[2];
+      st[0] = st[0] - 1000 * (st[0] / 1000);
+      if (st[1] >= 5)
+      {
+         st[1] = 0;
+         continue;
+      }
+      break;
+   }
+   return (st[0] - 500) / 1001.0;
+}
+
+/* DMAKE (c_dblat3.f): type 0 'GE', 1 'SY', 2 'TR'; fills the NMAX-leading
+ * A (m x n) and the packed-by-lda copy AA, the part that must not be
+ * referenced set to the rogue value -1e10. */
+void orc_blat_dmake(int type, int uplo, int diag, int m, int n, double *A, int nmax,
+                    double *AA, int lda, int *st, double transl)
+{
+   const int gen = type == 0, sym = type == 1, tri = type == 2;
+   const int upper = (sym || tri) && uplo == Upper, lower = (sym || tri) && uplo == Lower;
+   const int unit = tri && diag == Unit;
+   int i, j;
+   for (j = 0; j < n; j++)
+   {
+      for (i = 0; i < m; i++)
+         if (gen || (upper && i <= j) || (lower && i >= j))
+         {
+            A[i + (size_t)j * nmax] = orc_blat_dbeg(st) + transl;
+            if (i != j)
+            {
+               if (n > 3 && j + 1 == n / 2) A[i + (size_t)j * nmax] = 0.0;
+               if (sym) A[j + (size_t)i * nmax] = A[i + (size_t)j * nmax];
+               else if (tri) A[j + (size_t)i * nmax] = 0.0;
+            }
+         }
+      if (tri) A[j + (size_t)j * nmax] += 1.0;
+      if (unit) A[j + (size_t)j * nmax] = 1.0;
+   }
+   for (j = 0; j < n; j++)
+   {
+      int ibeg = 0, iend = m;  /* rows [ibeg, iend) copied, the rest rogue */
+      if (!gen)
+      {
+         if (upper)
+         {
+            ibeg = 0;
+            iend = unit ? j : j + 1;
+         }
+         else
+         {
+            ibeg = unit ? j + 1 : j;
+            iend = n;
+         }
+      }
+      for (i = 0; i < lda; i++)
+         AA[i + (size_t)j * lda] = (i >= ibeg && i < iend) ? A[i + (size_t)j * nmax] : -1.0e10;
+   }
+}
+
+/* DMMCH (c_dblat3.f): recomputes alpha op(A) op(B) + beta C column by column
+ * with gauges G = |alpha| |op(A)| |op(B)| + |beta| |C| and compares with CC:
+ * err_j = max_i |CT_i - CC_ij| / eps (divided by G_i when G_i != 0).  As in
+ * the Fortran, *err is the last column's ratio and the result is fatal (1)
+ * when some err_j * sqrt(eps) >= 1; *errmax gets the largest err_j. */
+int orc_blat_dmmch(int ta, int tb, int m, int n, int kk, double alpha, const double *A, int lda,
+                   const double *B, int ldb, double beta, const double *C, int ldc,
+                   const double *CC, int ldcc, double eps, double *err, double *errmax)
+{
+   const int trana = ta != NoTrans, tranb = tb != NoTrans;
+   double *ct = (double *)malloc(sizeof(double) * (m > 0 ? m : 1));
+   double *g = (double *)malloc(sizeof(double) * (m > 0 ? m : 1));
+   int i, j, k, fatal = 0;
+   *err = 0.0;
+   *errmax = 0.0;
+   for (j = 0; j < n && !fatal; j++)
+   {
+      for (i = 0; i < m; i++) ct[i] = g[i] = 0.0;
+      for (k = 0; k < kk; k++)
+         for (i = 0; i < m; i++)
+         {
+            const double a = trana ? A[k + (size_t)i * lda] : A[i + (size_t)k * lda];
+            const double b = tranb ? B[j + (size_t)k * ldb] : B[k + (size_t)j * ldb];
+            ct[i] += a * b;
+            g[i] += fabs(a) * fabs(b);
+         }
+      for (i = 0; i < m; i++)
+      {
+         ct[i] = alpha * ct[i] + beta * C[i + (size_t)j * ldc];
+         g[i] = fabs(alpha) * g[i] + fabs(beta) * fabs(C[i + (size_t)j * ldc]);
+      }
+      *err = 0.0;
+      for (i = 0; i < m; i++)
+      {
+         double erri = fabs(ct[i] - CC[i + (size_t)j * ldcc]) / eps;
+         if (g[i] != 0.0) erri /= g[i];
+         if (erri > *err || erri != erri) *err = erri != erri ? HUGE_VAL : erri;
+         if (*err * sqrt(eps) >= 1.0)
+         {
+            fatal = 1;
+            break;
+         }
+      }
+      if (*err > *errmax) *errmax = *err;
+   }
+   free(ct);
+   free(g);
+   return fatal;
+}
+
+/* Wall-clock seconds of orc_refgemm (NN, alpha=beta=1) on C(:,0:ncols),
+ * the columns split into nthreads contiguous slabs, one POSIX thread each
+ * (each slab is the same textbook loop; the threads share nothing they
+ * write). */
+struct slab_arg { int M, n, K, lda, ldb, ldc; const double *A, *B; double *C; };
+static void *slab_run(void *p)
+{
+   const struct slab_arg *a = (const struct slab_arg *)p;
+   if (a->n > 0)
+      orc_refgemm(NoTrans, NoTrans, a->M, a->n, a->K, 1.0, a->A, a->lda, a->B, a->ldb, 1.0,
+                  a->C, a->ldc);
+   return NULL;
+}
+double orc_time_refgemm_slab_mt(int M, int ncols, int K, const double *A, int lda,
+                                const double *B, int ldb, double *C, int ldc, int nthreads)
+{
+   struct timespec t0, t1;
+   pthread_t th[256];
+   struct slab_arg arg[256];
+   int t, j0 = 0;
+   if (nthreads < 1) nthreads = 1;
+   if (nthreads > 256) nthreads = 256;
+   for (t = 0; t < nthreads; t++)
+   {
+      const int n = ncols / nthreads + (t < ncols % nthreads);
+      arg[t].M = M; arg[t].n = n; arg[t].K = K;
+      arg[t].lda = lda; arg[t].ldb = ldb; arg[t].ldc = ldc;
+      arg[t].A = A; arg[t].B = B + (size_t)j0 * ldb; arg[t].C = C + (size_t)j0 * ldc;
+      j0 += n;
+   }
+   clock_gettime(CLOCK_MONOTONIC, &t0);
+   for (t = 1; t < nthreads; t++) pthread_create(&th[t], NULL, slab_run, &arg[t]);
+   slab_run(&arg[0]);
+   for (t = 1; t < nthreads; t++) pthread_join(th[t], NULL);
+   clock_gettime(CLOCK_MONOTONIC, &t1);
+   return (t1.tv_sec - t0.tv_sec) + 1e-9 * (t1.tv_nsec - t0.tv_nsec);
 }
 
 /* Wall-clock seconds of orc_refgemm (NN, alpha=beta=1) on C(:,0:ncols) */

@@ -76,6 +76,22 @@ double orc_slv_resid(int N, int NRHS, double nrmA, const double *XC,
 int    orc_npvts(int N, const int *ipiv);
 
 /* --- CPU baseline helper: wall time of refgemm on a column slab --- */
+void orc_reftrmm(int side, int uplo, int ta, int diag, int M, int N, double alpha,
+                 const double *A, int lda, double *B, int ldb);
+void orc_refsymm(int side, int uplo, int M, int N, double alpha, const double *A, int lda,
+                 const double *B, int ldb, double beta, double *C, int ldc);
+void orc_refsyr2k(int uplo, int trans, int N, int K, double alpha, const double *A, int lda,
+                  const double *B, int ldb, double beta, double *C, int ldc);
+int orc_reftrtri(int uplo, int diag, int N, double *A, int lda);
+void orc_reflauum(int uplo, int N, double *A, int lda);
+double orc_blat_dbeg(int *st);
+void orc_blat_dmake(int type, int uplo, int diag, int m, int n, double *A, int nmax,
+                    double *AA, int lda, int *st, double transl);
+int orc_blat_dmmch(int ta, int tb, int m, int n, int kk, double alpha, const double *A, int lda,
+                   const double *B, int ldb, double beta, const double *C, int ldc,
+                   const double *CC, int ldcc, double eps, double *err, double *errmax);
+double orc_time_refgemm_slab_mt(int M, int ncols, int K, const double *A, int lda,
+                                const double *B, int ldb, double *C, int ldc, int nthreads);
 double orc_time_refgemm_slab(int M, int ncols, int K, const double *A,
                              int lda, const double *B, int ldb, double *C,
                              int ldc);

@@ -74,6 +74,17 @@ def lib():
             "orc_slv_resid": (_d, [_i, _i, _d, _dp, _i, _dp, _i]),
             "orc_npvts": (_i, [_i, _ip]),
             "orc_time_refgemm_slab": (_d, [_i, _i, _i, _dp, _i, _dp, _i, _dp, _i]),
+            "orc_time_refgemm_slab_mt": (_d, [_i, _i, _i, _dp, _i, _dp, _i, _dp, _i, _i]),
+            "orc_reftrmm": (None, [_i, _i, _i, _i, _i, _i, _d, _dp, _i, _dp, _i]),
+            "orc_refsymm": (None, [_i, _i, _i, _i, _d, _dp, _i, _dp, _i, _d, _dp, _i]),
+            "orc_refsyr2k": (None, [_i, _i, _i, _i, _d, _dp, _i, _dp, _i, _d, _dp, _i]),
+            "orc_reftrtri": (_i, [_i, _i, _i, _dp, _i]),
+            "orc_reflauum": (None, [_i, _i, _dp, _i]),
+            "orc_blat_dbeg": (_d, [_ip]),
+            "orc_blat_dmake": (None, [_i, _i, _i, _i, _i, _dp, _i, _dp, _i, _ip, _d]),
+            "orc_blat_dmmch": (_i, [_i, _i, _i, _i, _i, _d, _dp, _i, _dp, _i, _d, _dp, _i, _dp,
+                                    _i, _d, ctypes.POINTER(ctypes.c_double),
+                                    ctypes.POINTER(ctypes.c_double)]),
         }
         for name, (res, args) in sig.items():
             f = getattr(L, name)
@@ -169,6 +180,29 @@ def reftrsm(side, uplo, ta, diag, m, n, alpha, a, b):
     lib().orc_reftrsm(side, uplo, ta, diag, m, n, alpha, _p(a), _ld(a), _p(b), _ld(b))
 
 
+def reftrmm(side, uplo, ta, diag, m, n, alpha, a, b):
+    lib().orc_reftrmm(side, uplo, ta, diag, m, n, alpha, _p(a), _ld(a), _p(b), _ld(b))
+
+
+def refsymm(side, uplo, m, n, alpha, a, b, beta, c):
+    lib().orc_refsymm(side, uplo, m, n, alpha, _p(a), _ld(a), _p(b), _ld(b), beta, _p(c), _ld(c))
+
+
+def refsyr2k(uplo, trans, n, k, alpha, a, b, beta, c):
+    lib().orc_refsyr2k(uplo, trans, n, k, alpha, _p(a), _ld(a), _p(b), _ld(b), beta, _p(c),
+                       _ld(c))
+
+
+def reftrtri(uplo, diag, a, n=None):
+    n = a.shape[1] if n is None else n
+    return lib().orc_reftrtri(uplo, diag, n, _p(a), _ld(a))
+
+
+def reflauum(uplo, a, n=None):
+    n = a.shape[1] if n is None else n
+    lib().orc_reflauum(uplo, n, _p(a), _ld(a))
+
+
 def idamax(x):
     x = np.ascontiguousarray(x, dtype=np.float64)
     return lib().orc_idamax(x.size, _p(x), 1)
@@ -245,3 +279,51 @@ def npvts(ipiv):
 
 def time_refgemm_slab(m, ncols, k, a, b, c):
     return lib().orc_time_refgemm_slab(m, ncols, k, _p(a), _ld(a), _p(b), _ld(b), _p(c), _ld(c))
+
+
+def time_refgemm_slab_mt(m, ncols, k, a, b, c, nthreads):
+    return lib().orc_time_refgemm_slab_mt(m, ncols, k, _p(a), _ld(a), _p(b), _ld(b), _p(c), _ld(c),
+                                          nthreads)
+
+
+# ---- netlib CBLAS level-3 tester (c_dblat3.f) machinery --------------------
+
+class BlatGen:
+    """DBEG's state; reset=True restarts the sequence (RESET in the Fortran)."""
+
+    def __init__(self):
+        self.st = np.array([0, 0, 0, 1], dtype=np.int32)
+
+    def reset(self):
+        self.st[3] = 1
+
+    def dbeg(self):
+        return lib().orc_blat_dbeg(_pi(self.st))
+
+
+BLAT_GE, BLAT_SY, BLAT_TR = 0, 1, 2
+
+
+def blat_dmake(gen, typ, uplo, diag, m, n, nmax, lda, transl=0.0):
+    """DMAKE: returns (A as an nmax-leading Fortran array, AA with leading
+    dimension lda)."""
+    a = fmat(max(nmax, 1), max(n, 1))
+    aa = fmat(max(lda, 1), max(n, 1))
+    lib().orc_blat_dmake(typ, uplo, diag, m, n, _p(a), max(nmax, 1), _p(aa), max(lda, 1),
+                         _pi(gen.st), transl)
+    return a, aa
+
+
+def blat_eps():
+    """c_dblat3's EPS: twice LAPACK's dlamch('E') (2^-53), i.e. 2^-52."""
+    return 2.0 * 2.0 ** -53
+
+
+def blat_dmmch(ta, tb, m, n, kk, alpha, a, b, beta, c, cc, eps=None):
+    """DMMCH: (fatal, err of the last column, largest column err)."""
+    eps = blat_eps() if eps is None else eps
+    err, emax = ctypes.c_double(0.0), ctypes.c_double(0.0)
+    fatal = lib().orc_blat_dmmch(ta, tb, m, n, kk, alpha, _p(a), _ld(a), _p(b), _ld(b), beta,
+                                 _p(c), _ld(c), _p(cc), _ld(cc), eps, ctypes.byref(err),
+                                 ctypes.byref(emax))
+    return bool(fatal), err.value, emax.value

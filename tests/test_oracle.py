@@ -392,3 +392,203 @@ def test_params_fixture_present():
         p = json.load(f)
     assert p["cblat3"]["N"] == [0, 1, 2, 3, 5, 9]
     assert p["thresholds"]["l3blastst"] == 50.0
+
+
+# ---- TRMM, SYMM, SYR2K, TRTRI, LAUUM against numpy ------------------------
+
+def _np_op_tri(a, n, uplo, ta, diag):
+    t = np.tril(a[:n, :n]) if uplo == O.Lower else np.triu(a[:n, :n])
+    if diag == O.Unit:
+        np.fill_diagonal(t, 1.0)
+    return t.T if ta != O.NoTrans else t
+
+
+@pytest.mark.parametrize("side", [O.Left, O.Right])
+@pytest.mark.parametrize("uplo", [O.Lower, O.Upper])
+@pytest.mark.parametrize("ta", [O.NoTrans, O.Trans])
+@pytest.mark.parametrize("diag", [O.NonUnit, O.Unit])
+def test_reftrmm_against_numpy(orc, side, uplo, ta, diag):
+    r = rng(20)
+    m, n = 7, 5
+    na = m if side == O.Left else n
+    a = np.asfortranarray(r.uniform(-1, 1, (na + 2, na)))
+    a[np.diag_indices(na)] = np.nan if diag == O.Unit else 2.0  # unit diagonal never read
+    b0 = np.asfortranarray(r.uniform(-1, 1, (m + 1, n)))
+    b = b0.copy(order="F")
+    O.reftrmm(side, uplo, ta, diag, m, n, 1.5, a, b)
+    t = _np_op_tri(a, na, uplo, ta, diag)
+    want = 1.5 * (t @ b0[:m] if side == O.Left else b0[:m] @ t)
+    assert np.allclose(b[:m], want, rtol=1e-14, atol=1e-14)
+    assert np.array_equal(b[m:], b0[m:])
+
+
+@pytest.mark.parametrize("side", [O.Left, O.Right])
+@pytest.mark.parametrize("uplo", [O.Lower, O.Upper])
+def test_refsymm_against_numpy(orc, side, uplo):
+    r = rng(21)
+    m, n = 6, 4
+    na = m if side == O.Left else n
+    a = np.asfortranarray(r.uniform(-1, 1, (na, na)))
+    s = np.tril(a) + np.tril(a, -1).T if uplo == O.Lower else np.triu(a) + np.triu(a, 1).T
+    if uplo == O.Lower:  # the other triangle is never read
+        a[np.triu_indices(na, 1)] = np.nan
+    else:
+        a[np.tril_indices(na, -1)] = np.nan
+    b = np.asfortranarray(r.uniform(-1, 1, (m, n)))
+    c0 = np.asfortranarray(r.uniform(-1, 1, (m, n)))
+    c = c0.copy(order="F")
+    O.refsymm(side, uplo, m, n, 0.7, a, b, 1.3, c)
+    want = 0.7 * (s @ b if side == O.Left else b @ s) + 1.3 * c0
+    assert np.allclose(c, want, rtol=1e-14, atol=1e-14)
+
+
+@pytest.mark.parametrize("uplo", [O.Lower, O.Upper])
+@pytest.mark.parametrize("trans", [O.NoTrans, O.Trans])
+def test_refsyr2k_against_numpy(orc, uplo, trans):
+    r = rng(22)
+    n, k = 6, 3
+    shape = (n, k) if trans == O.NoTrans else (k, n)
+    a = np.asfortranarray(r.uniform(-1, 1, shape))
+    b = np.asfortranarray(r.uniform(-1, 1, shape))
+    c0 = np.asfortranarray(r.uniform(-1, 1, (n, n)))
+    c = c0.copy(order="F")
+    O.refsyr2k(uplo, trans, n, k, 0.7, a, b, 0.0 if uplo == O.Lower else 1.3, c)
+    oa, ob = (a, b) if trans == O.NoTrans else (a.T, b.T)
+    beta = 0.0 if uplo == O.Lower else 1.3
+    full = 0.7 * (oa @ ob.T + ob @ oa.T) + beta * c0
+    mask = np.tril(np.ones((n, n), bool)) if uplo == O.Lower else np.triu(np.ones((n, n), bool))
+    assert np.allclose(c[mask], full[mask], rtol=1e-14, atol=1e-14)
+    assert np.array_equal(c[~mask], c0[~mask])
+
+
+@pytest.mark.parametrize("uplo", [O.Lower, O.Upper])
+@pytest.mark.parametrize("diag", [O.NonUnit, O.Unit])
+def test_reftrtri_against_numpy(orc, uplo, diag):
+    r = rng(23)
+    n = 9
+    a = np.asfortranarray(r.uniform(-1, 1, (n, n)) / n + np.diag(r.uniform(1, 2, n)))
+    a0 = a.copy(order="F")
+    assert O.reftrtri(uplo, diag, a) == 0
+    t = _np_op_tri(a0, n, uplo, O.NoTrans, diag)
+    inv = np.linalg.inv(t)
+    mask = np.tril(np.ones((n, n), bool)) if uplo == O.Lower else np.triu(np.ones((n, n), bool))
+    if diag == O.Unit:
+        mask &= ~np.eye(n, dtype=bool)
+    assert np.allclose(a[mask], inv[mask], rtol=1e-13, atol=1e-13)
+    assert np.array_equal(a[~mask], a0[~mask])
+    # a zero on a NonUnit diagonal: its 1-based index, A untouched
+    z = a0.copy(order="F")
+    z[4, 4] = 0.0
+    z0 = z.copy(order="F")
+    assert O.reftrtri(uplo, O.NonUnit, z) == 5 and np.array_equal(z, z0)
+
+
+@pytest.mark.parametrize("uplo", [O.Lower, O.Upper])
+def test_reflauum_against_numpy(orc, uplo):
+    r = rng(24)
+    n = 8
+    a = np.asfortranarray(r.uniform(-1, 1, (n, n)))
+    a0 = a.copy(order="F")
+    O.reflauum(uplo, a)
+    if uplo == O.Lower:
+        l = np.tril(a0)
+        want, mask = l.T @ l, np.tril(np.ones((n, n), bool))
+    else:
+        u = np.triu(a0)
+        want, mask = u @ u.T, np.triu(np.ones((n, n), bool))
+    assert np.allclose(a[mask], want[mask], rtol=1e-14, atol=1e-14)
+    assert np.array_equal(a[~mask], a0[~mask])
+
+
+# ---- c_dblat3's machinery (DBEG, DMAKE, DMMCH) ------------------------------
+
+def test_blat_dmmch_exact_data_self_check(orc):
+    # the self-check c_dblat3 runs before its tests
+    # (AtlasBase/Clint/misc/CBLATST/c_dblat3.f:212-280): AB(i,j) =
+    # max(i-j+1, 0), a vector j, and the closed-form product
+    # CC(j) = j((j+1)j)/2 - ((j+1)j(j-1))/3 must give ERR = 0 for all four
+    # transpose pairs
+    n, nmax = 32, 65
+    ab = np.zeros((nmax, 2 * nmax + 1), order="F")
+    for j in range(1, n + 1):
+        for i in range(1, n + 1):
+            ab[i - 1, j - 1] = max(i - j + 1, 0)
+        ab[j - 1, nmax] = j
+        ab[0, nmax + j] = j
+    c = np.zeros((nmax, 1), order="F")
+    a = np.asfortranarray(ab[:, :nmax])
+    bcol = np.asfortranarray(ab[:, nmax:nmax + 1])          # B(k, 1) = k
+    brow = np.asfortranarray(ab[:1, nmax + 1:nmax + 1 + n])  # B(1, k) = k (read as B^T)
+    cc = np.zeros((nmax, 1), order="F")
+    for j in range(1, n + 1):
+        cc[j - 1, 0] = j * ((j + 1) * j) // 2 - ((j + 1) * j * (j - 1)) // 3
+    eps = O.blat_eps()
+    fatal, err, emax = O.blat_dmmch(O.NoTrans, O.NoTrans, n, 1, n, 1.0, a, bcol, 0.0, c, cc, eps)
+    assert not fatal and err == 0.0 and emax == 0.0
+    fatal, err, _ = O.blat_dmmch(O.NoTrans, O.Trans, n, 1, n, 1.0, a, brow, 0.0, c, cc, eps)
+    assert not fatal and err == 0.0
+    # reversed vector with transposed A
+    for j in range(1, n + 1):
+        bcol[j - 1, 0] = n - j + 1
+        brow[0, j - 1] = n - j + 1
+    for j in range(1, n + 1):
+        cc[n - j, 0] = j * ((j + 1) * j) // 2 - ((j + 1) * j * (j - 1)) // 3
+    fatal, err, _ = O.blat_dmmch(O.Trans, O.NoTrans, n, 1, n, 1.0, a, bcol, 0.0, c, cc, eps)
+    assert not fatal and err == 0.0
+    fatal, err, _ = O.blat_dmmch(O.Trans, O.Trans, n, 1, n, 1.0, a, brow, 0.0, c, cc, eps)
+    assert not fatal and err == 0.0
+    # a planted error: the ratio is |d| / (eps * gauge) for that row, and a
+    # large one is fatal (err * sqrt(eps) >= 1).  Forward data, NoTrans:
+    # row 4 has gauge sum_k |A(4,k)| k = 20 = its exact value.
+    for j in range(1, n + 1):
+        bcol[j - 1, 0] = j
+        cc[j - 1, 0] = j * ((j + 1) * j) // 2 - ((j + 1) * j * (j - 1)) // 3
+    assert cc[3, 0] == 20.0
+    bad = cc.copy(order="F")
+    bad[3, 0] += 20.0 * 4.0 * eps
+    fatal, err, _ = O.blat_dmmch(O.NoTrans, O.NoTrans, n, 1, n, 1.0, a, bcol, 0.0, c, bad, eps)
+    assert not fatal and err == pytest.approx(4.0, rel=1e-12)
+    bad[3, 0] += 1.0
+    assert O.blat_dmmch(O.NoTrans, O.NoTrans, n, 1, n, 1.0, a, bcol, 0.0, c, bad, eps)[0]
+
+
+def test_blat_dbeg_and_dmake_structure(orc):
+    g = O.BlatGen()
+    vals = [g.dbeg() for _ in range(400)]
+    # values (i - 500) / 1001 with i in [0, 999]
+    ints = [round(v * 1001 + 500) for v in vals]
+    assert all(0 <= i <= 999 for i in ints)
+    assert all(abs((i - 500) / 1001.0 - v) < 1e-15 for i, v in zip(ints, vals))
+    # the sequence: i <- 891 i mod 1000 from 7, every fifth step skipped
+    i, ic, want = 7, 0, []
+    while len(want) < 400:
+        ic += 1
+        i = (i * 891) % 1000
+        if ic >= 5:
+            ic = 0
+            i = (i * 891) % 1000
+        want.append(i)
+    assert ints == want
+    g.reset()
+    assert g.dbeg() == vals[0]
+    # DMAKE: symmetric copy, zero column n/2 off the diagonal, triangles
+    # zeroed, +1 on a triangular diagonal, rogue padding
+    n, nmax, lda = 6, 8, 7
+    g.reset()
+    s, ss = O.blat_dmake(g, O.BLAT_SY, O.Lower, O.NonUnit, n, n, nmax, lda)
+    assert np.array_equal(s[:n, :n], s[:n, :n].T)
+    col = n // 2 - 1
+    assert np.all(s[col + 1:n, col] == 0.0)  # the generated part of column n/2
+    assert np.all(ss[n:lda, :n] == -1.0e10)
+    assert np.all(ss[np.triu_indices(n, 1)] == -1.0e10)          # upper part is rogue
+    g.reset()
+    t, tt = O.blat_dmake(g, O.BLAT_TR, O.Upper, O.Unit, n, n, nmax, lda)
+    assert np.all(np.diag(t[:n, :n]) == 1.0)
+    assert np.all(t[:n, :n][np.tril_indices(n, -1)] == 0.0)
+    assert np.all(np.diag(tt[:n, :n]) == -1.0e10)                # unit diagonal is rogue
+    g.reset()
+    t, _ = O.blat_dmake(g, O.BLAT_TR, O.Lower, O.NonUnit, n, n, nmax, lda)
+    g.reset()
+    gen = O.BlatGen()
+    first = gen.dbeg()
+    assert t[0, 0] == pytest.approx(first + 1.0)
