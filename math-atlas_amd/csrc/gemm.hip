@@ -290,8 +290,8 @@ gemm_mfma(int M, int N, int K, S alpha, const S* A, long lda, const S* B, long l
   }
 }
 
-// FP64 C = alpha*A*B + beta*C for A and B not transposed, both 16-byte
-// aligned with even leading dimensions, K a multiple of 32: the A operand
+// FP64 C = alpha*op(A)*op(B) + beta*C for A and B 16-byte aligned with
+// even leading dimensions, K a multiple of 32: the A operand
 // goes from memory straight into the MFMA operand registers, and only B is
 // staged through (double-buffered) LDS.  Halving the LDS stores per K step
 // is what lifts this form above gemm_mfma (measured on MI355X at
@@ -310,56 +310,105 @@ gemm_mfma(int M, int N, int K, S alpha, const S* A, long lda, const S* B, long l
 // waits stay exact.  Rows past M and columns past N are clamped on load and
 // masked on store.  One barrier per K tile, placed before the last MFMA
 // step so that step covers the next tile's first LDS reads.
+//
+// AT: op(A) = A^T, each lane loading its rows' k runs instead of row pairs.
+// BT selects op(B): false, B is K x N (a tile column of 16 k is 128
+// contiguous bytes, loaded as 8 column chunks per wave and transposed into
+// LDS by two 8-byte stores); true, op(B) = B^T with B N x K (a tile row of
+// 128 n is contiguous, loaded and stored to LDS 16 bytes per lane).  mode
+// 1 / 2 computes only the tiles and elements on or below / above the
+// diagonal (SYRK and the Cholesky update), enumerated as gemm_mfma does.
 constexpr int NN_DA = 2;  // K tiles of A in flight
 constexpr int kBufDword3 = 0x00020000;  // raw buffer, 32-bit dwords (gfx9 family)
-__global__ void __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(2, 2)))
-gemm_nn_f64(int M, int N, int K, double alpha, const double* __restrict__ A, long lda,
-            const double* B, long ldb, double beta, double* C, long ldc, int vecc,
-            const int* __restrict__ skip) {
+template <bool AT, bool BT>
+__device__ __forceinline__ void gemm_rega_f64(int M, int N, int K, double alpha,
+                                              const double* __restrict__ A, long lda,
+                                              const double* B, long ldb, double beta, double* C,
+                                              long ldc, int vecc, int mode) {
   typedef Mfma<double> MF;
   typedef double d2 __attribute__((ext_vector_type(2)));
   constexpr int LDB_S = BN + PADB;
-  if (skip && *skip) return;
   __shared__ __attribute__((aligned(16))) double Bs[2][BK * LDB_S];
-  const int tilesM = (M + BM - 1) / BM, tilesN = (N + BN - 1) / BN, ntiles = tilesM * tilesN;
+  const int tilesM = (M + BM - 1) / BM, tilesN = (N + BN - 1) / BN;
+  const int ntiles = mode == 0 ? tilesM * tilesN : tri_tiles(mode, tilesM, tilesN);
   const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
   const int wm0 = (wave >> 1) * WM, wn0 = (wave & 1) * WN;
   const int fx = lane & 15, fk = lane >> 4;
   const int nk = K / BK;
   for (int bid = blockIdx.x; bid < ntiles; bid += gridDim.x) {
     int tm, tn;
-    tile_of(bid, ntiles, tilesM, tilesN, tm, tn);
+    if (mode == 0) {
+      tile_of(bid, ntiles, tilesM, tilesN, tm, tn);
+    } else {
+      const int per = ntiles / 8, rem = ntiles % 8, xcd = bid % 8, local = bid / 8;
+      const int id = !ATLG_GEMM_XCD ? bid
+                     : (xcd < rem) ? xcd * (per + 1) + local
+                                   : rem * (per + 1) + (xcd - rem) * per + local;
+      tri_tile_of(id, mode, tilesM, tilesN, tm, tn);
+    }
     const int m0 = tm * BM, n0 = tn * BN;
     // Addresses: a wave-uniform base per tile and K step plus a 32-bit
     // per-lane element offset inside the tile.  A: this lane's row pairs,
     // clamped so a pair starts at most at M-1 (lda is even, so the pair
-    // stays inside the allocation).
-    const double* At = A + m0;
-    unsigned aoff[2];  // bytes
+    // stays inside the allocation).  A^T: the lane's four rows (clamped to
+    // M-1), each holding k = 4fk .. 4fk+3 contiguously; the MFMA steps then
+    // take k = 4fk + ks (any k order serves, as long as B's matches).
+    const double* At = AT ? A + (size_t)m0 * lda : A + m0;
+    unsigned aoff[AT ? 4 : 2];  // bytes
+    if (AT) {
 #pragma unroll
-    for (int h = 0; h < 2; h++)
-      aoff[h] = 8u * (unsigned)(min(wm0 + 32 * h + 2 * fx, ((M - 1) & ~1) - m0) + fk * (int)lda);
-    d2 ar[NN_DA][4][2];
-    auto aload = [&](int d, int ks, int k0) {
-      const __amdgpu_buffer_rsrc_t r = __builtin_amdgcn_make_buffer_rsrc(
-          const_cast<double*>(At + (size_t)(k0 + 4 * ks) * lda), 0, 0x7fffffff, kBufDword3);
+      for (int i = 0; i < 4; i++)
+        aoff[i] = 8u * (unsigned)(min(wm0 + 32 * (i >> 1) + 2 * fx + (i & 1), M - 1 - m0) *
+                                      (int)lda + 4 * fk);
+    } else {
 #pragma unroll
       for (int h = 0; h < 2; h++)
-        ar[d][ks][h] = __builtin_bit_cast(d2, __builtin_amdgcn_raw_buffer_load_b128(r, aoff[h], 0, 0));
+        aoff[h] = 8u * (unsigned)(min(wm0 + 32 * h + 2 * fx, ((M - 1) & ~1) - m0) + fk * (int)lda);
+    }
+    // A registers per K tile: NoTrans, slot 2ks + h holds rows of tiles 2h,
+    // 2h+1 at step ks; A^T, slot 4p + i holds tile i's k pair p (steps 2p,
+    // 2p+1)
+    d2 ar[NN_DA][8];
+    auto aval = [&](int d, int ks, int i) -> double {
+      return AT ? ar[d][(ks >> 1) * 4 + i][ks & 1] : ar[d][ks * 2 + (i >> 1)][i & 1];
     };
-    // B: 4 chunks of 8 columns x 16 k per wave, 16 bytes per lane; columns
-    // past N clamped to N-1
+    // loads the registers step ks of K tile slot d has just consumed
+    auto aload = [&](int d, int ks, int k0) {
+      if (AT) {
+        if (ks & 1) {
+          const int p = ks >> 1;
+          const __amdgpu_buffer_rsrc_t r = __builtin_amdgcn_make_buffer_rsrc(
+              const_cast<double*>(At + k0 + 2 * p), 0, 0x7fffffff, kBufDword3);
+#pragma unroll
+          for (int i = 0; i < 4; i++)
+            ar[d][4 * p + i] =
+                __builtin_bit_cast(d2, __builtin_amdgcn_raw_buffer_load_b128(r, aoff[i], 0, 0));
+        }
+      } else {
+        const __amdgpu_buffer_rsrc_t r = __builtin_amdgcn_make_buffer_rsrc(
+            const_cast<double*>(At + (size_t)(k0 + 4 * ks) * lda), 0, 0x7fffffff, kBufDword3);
+#pragma unroll
+        for (int h = 0; h < 2; h++)
+          ar[d][2 * ks + h] =
+              __builtin_bit_cast(d2, __builtin_amdgcn_raw_buffer_load_b128(r, aoff[h], 0, 0));
+      }
+    };
+    // B (not transposed): 4 chunks of 8 columns x 16 k per wave, 16 bytes
+    // per lane, columns past N clamped to N-1.  B^T: k rows 4i + wave, 2
+    // columns per lane, the pair start clamped to at most N-1 (ldb is even,
+    // so the pair stays inside the allocation).
     constexpr int B_PER = BN / 8 / (NT / 64);
-    const double* Bt = B + (size_t)n0 * ldb;
+    const double* Bt = BT ? B + n0 : B + (size_t)n0 * ldb;
     unsigned boff[B_PER];  // bytes
 #pragma unroll
     for (int i = 0; i < B_PER; i++)
-      boff[i] = 8u * (unsigned)(min((wave * B_PER + i) * 8 + (lane >> 3), N - 1 - n0) * (int)ldb +
-                                2 * (lane & 7));
+      boff[i] = BT ? 8u * (unsigned)(min(2 * lane, ((N - 1) & ~1) - n0) + (4 * i + wave) * (int)ldb)
+                   : 8u * (unsigned)(min((wave * B_PER + i) * 8 + (lane >> 3), N - 1 - n0) *
+                                         (int)ldb + 2 * (lane & 7));
     d2 rb[B_PER];
     auto bload = [&](int k0) {
-      const __amdgpu_buffer_rsrc_t r =
-          __builtin_amdgcn_make_buffer_rsrc(const_cast<double*>(Bt + k0), 0, 0x7fffffff, kBufDword3);
+      const __amdgpu_buffer_rsrc_t r = __builtin_amdgcn_make_buffer_rsrc(
+          const_cast<double*>(BT ? Bt + (size_t)k0 * ldb : Bt + k0), 0, 0x7fffffff, kBufDword3);
 #pragma unroll
       for (int i = 0; i < B_PER; i++)
         rb[i] = __builtin_bit_cast(d2, __builtin_amdgcn_raw_buffer_load_b128(r, boff[i], 0, 0));
@@ -367,15 +416,20 @@ gemm_nn_f64(int M, int N, int K, double alpha, const double* __restrict__ A, lon
     auto bstore = [&](int buf) {
 #pragma unroll
       for (int i = 0; i < B_PER; i++) {
-        const int n = (wave * B_PER + i) * 8 + (lane >> 3), k = 2 * (lane & 7);
-        Bs[buf][k * LDB_S + n] = rb[i][0];
-        Bs[buf][(k + 1) * LDB_S + n] = rb[i][1];
+        if (BT) {
+          *reinterpret_cast<d2*>(&Bs[buf][(4 * i + wave) * LDB_S + 2 * lane]) = rb[i];
+        } else {
+          const int n = (wave * B_PER + i) * 8 + (lane >> 3), k = 2 * (lane & 7);
+          Bs[buf][k * LDB_S + n] = rb[i][0];
+          Bs[buf][(k + 1) * LDB_S + n] = rb[i][1];
+        }
       }
     };
     double bf[2][TN];
     auto bread = [&](int buf, int ks, int f) {
 #pragma unroll
-      for (int j = 0; j < TN; j++) bf[f][j] = Bs[buf][(ks * 4 + fk) * LDB_S + wn0 + j * 16 + fx];
+      for (int j = 0; j < TN; j++)
+        bf[f][j] = Bs[buf][(AT ? 4 * fk + ks : ks * 4 + fk) * LDB_S + wn0 + j * 16 + fx];
     };
     MF::acc_t acc[TM][TN];
 #pragma unroll
@@ -386,7 +440,7 @@ gemm_nn_f64(int M, int N, int K, double alpha, const double* __restrict__ A, lon
 #pragma unroll
       for (int i = 0; i < TM; i++)
 #pragma unroll
-        for (int j = 0; j < TN; j++) acc[i][j] = MF::op(bf[f][j], ar[d][ks][i >> 1][i & 1], acc[i][j]);
+        for (int j = 0; j < TN; j++) acc[i][j] = MF::op(bf[f][j], aval(d, ks, i), acc[i][j]);
     };
     bload(0);
     bstore(0);
@@ -420,33 +474,38 @@ gemm_nn_f64(int M, int N, int K, double alpha, const double* __restrict__ A, lon
     }
 #undef ATLG_SB
     // epilogue: the rows of tiles 2h and 2h+1 are adjacent (16-byte C
-    // accesses when C allows them); beta == 0 never reads C
+    // accesses when C allows them and both rows are kept); beta == 0 never
+    // reads C
 #pragma unroll
     for (int h = 0; h < 2; h++)
 #pragma unroll
       for (int j = 0; j < TN; j++) {
         const int gi = m0 + wm0 + 32 * h + 2 * fx;
         double cv[4][2];
+        bool keep[4][2];
 #pragma unroll
         for (int r = 0; r < 4; r++) {
           const int gj = n0 + wn0 + j * 16 + MF::c_n(lane, r);
+#pragma unroll
+          for (int e = 0; e < 2; e++)
+            keep[r][e] = gi + e < M && gj < N &&
+                         (mode == 0 || (mode == 1 ? gi + e >= gj : gi + e <= gj));
           double* c = C + (size_t)gi + (size_t)gj * ldc;
           cv[r][0] = cv[r][1] = 0.0;
-          if (beta != 0.0 && gj < N) {
-            if (vecc && gi + 1 < M) {
+          if (beta != 0.0) {
+            if (vecc && keep[r][0] && keep[r][1]) {
               const d2 w = *reinterpret_cast<const d2*>(c);
               cv[r][0] = w[0];
               cv[r][1] = w[1];
             } else {
-              if (gi < M) cv[r][0] = c[0];
-              if (gi + 1 < M) cv[r][1] = c[1];
+              if (keep[r][0]) cv[r][0] = c[0];
+              if (keep[r][1]) cv[r][1] = c[1];
             }
           }
         }
 #pragma unroll
         for (int r = 0; r < 4; r++) {
           const int gj = n0 + wn0 + j * 16 + MF::c_n(lane, r);
-          if (gj >= N) continue;
           double* c = C + (size_t)gi + (size_t)gj * ldc;
           d2 o;
           o[0] = alpha * acc[2 * h][j][r];
@@ -455,16 +514,40 @@ gemm_nn_f64(int M, int N, int K, double alpha, const double* __restrict__ A, lon
             o[0] += beta * cv[r][0];
             o[1] += beta * cv[r][1];
           }
-          if (vecc && gi + 1 < M) {
+          if (vecc && keep[r][0] && keep[r][1]) {
             *reinterpret_cast<d2*>(c) = o;
           } else {
-            if (gi < M) c[0] = o[0];
-            if (gi + 1 < M) c[1] = o[1];
+            if (keep[r][0]) c[0] = o[0];
+            if (keep[r][1]) c[1] = o[1];
           }
         }
       }
     __syncthreads();  // the next tile's first B stores overwrite Bs[0]
   }
+}
+
+__global__ void __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(2, 2)))
+gemm_nn_f64(int M, int N, int K, double alpha, const double* __restrict__ A, long lda,
+            const double* B, long ldb, double beta, double* C, long ldc, int vecc, int mode,
+            const int* __restrict__ skip) {
+  if (skip && *skip) return;
+  gemm_rega_f64<false, false>(M, N, K, alpha, A, lda, B, ldb, beta, C, ldc, vecc, mode);
+}
+
+__global__ void __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(2, 2)))
+gemm_nt_f64(int M, int N, int K, double alpha, const double* __restrict__ A, long lda,
+            const double* B, long ldb, double beta, double* C, long ldc, int vecc, int mode,
+            const int* __restrict__ skip) {
+  if (skip && *skip) return;
+  gemm_rega_f64<false, true>(M, N, K, alpha, A, lda, B, ldb, beta, C, ldc, vecc, mode);
+}
+
+__global__ void __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(2, 2)))
+gemm_tn_f64(int M, int N, int K, double alpha, const double* __restrict__ A, long lda,
+            const double* B, long ldb, double beta, double* C, long ldc, int vecc, int mode,
+            const int* __restrict__ skip) {
+  if (skip && *skip) return;
+  gemm_rega_f64<true, false>(M, N, K, alpha, A, lda, B, ldb, beta, C, ldc, vecc, mode);
 }
 
 template <class S>
@@ -544,18 +627,21 @@ void gemm(hipStream_t s, int TA, int TB, long M, long N, long K, S alpha, const 
   const dim3 g((unsigned)nwg);
   const bool at = is_trans(TA), bt = is_trans(TB);
   const bool va = aligned16<S>(A, lda), vb = aligned16<S>(B, ldb);
-  // FP64 NoTrans x NoTrans with aligned operands, K a multiple of 16, the
-  // full C (no triangle) and C not one of the operands (TRSM's in-place
-  // diagonal apply): A straight to registers, B through LDS
-  // (per-lane byte offsets inside a tile are 32-bit: 128 columns of B, or
-  // 4 columns plus 128 rows of A)
-  if (std::is_same<S, double>::value && !at && !bt && va && vb && mode == 0 && K % (2 * BK) == 0 &&
-      C != A && C != B && 8L * (BN * ldb + BK) < (1L << 31) && 8L * (4 * lda + BM) < (1L << 31)) {
+  // FP64 with aligned operands, K a multiple of 32 and C not one of the
+  // operands (TRSM's in-place diagonal apply): A straight to registers, B
+  // through LDS (per-lane byte offsets inside a tile are 32-bit: 128
+  // columns or 16 rows of B, 4 columns plus 128 rows or 128 columns of A).
+  // Not for A^T x B^T, whose form needs more than 256 registers.
+  if (std::is_same<S, double>::value && !(at && bt) && va && vb && K % (2 * BK) == 0 &&
+      C != A && C != B &&
+      8L * (at ? BM * lda + BK : 4 * lda + BM) < (1L << 31) &&
+      8L * (bt ? BK * ldb + BN : BN * ldb + BK) < (1L << 31)) {
     const int vecc = aligned16<S>(C, ldc) ? 1 : 0;
-    hipLaunchKernelGGL(gemm_nn_f64, g, dim3(NT), 0, s, (int)M, (int)N, (int)K, (double)alpha,
+    hipLaunchKernelGGL(at ? gemm_tn_f64 : (bt ? gemm_nt_f64 : gemm_nn_f64),
+                       g, dim3(NT), 0, s, (int)M, (int)N, (int)K, (double)alpha,
                        reinterpret_cast<const double*>(A), lda,
                        reinterpret_cast<const double*>(B), ldb, (double)beta,
-                       reinterpret_cast<double*>(C), ldc, vecc, skip);
+                       reinterpret_cast<double*>(C), ldc, vecc, mode, skip);
     ATLG_LAUNCHED();
     return;
   }

@@ -37,10 +37,12 @@ def G():
     return atlas_gpu
 
 
-def gemm_case(G, ta, tb, m, n, k, alpha, beta, gap=0, order=None, nan_c=False):
+def gemm_case(G, ta, tb, m, n, k, alpha, beta, gap=0, order=None, nan_c=False, even=False):
     mA, nA = (m, k) if ta == O.NoTrans else (k, m)
     mB, nB = (k, n) if tb == O.NoTrans else (n, k)
     lda, ldb, ldc = max(mA, 1) + gap, max(mB, 1) + gap, max(m, 1) + gap
+    if even:  # A and B 16-byte aligned column by column (C keeps its gap)
+        lda, ldb = lda + (lda & 1), ldb + (ldb & 1)
     a = O.gegen(mA, nA, mA * nA + 513 * 7 + 90, lda=lda)
     b = O.gegen(mB, nB, mB * nB + 127 * 50 + 77, lda=ldb)
     c0 = O.gegen(m, n, m * n + 101 * 2 + 53, lda=ldc)
@@ -108,8 +110,50 @@ def test_gemm_nn_register_path_edges(G):
             (1, 1, 32, 2, 1.0, -1.1, False), (384, 384, 512, 0, 1.0, 1.0, False),
             (255, 129, 160, 2, -1.1, 0.0, True)):
         r, nbad, mx = gemm_case(G, O.NoTrans, O.NoTrans, m, n, k, alpha, beta, gap=gap,
-                                nan_c=nan_c)
+                                nan_c=nan_c, even=True)
         assert r < 50.0 and nbad == 0, (m, n, k, gap, alpha, beta, r, nbad, mx)
+
+
+@pytest.mark.parametrize("ta,tb", [(O.NoTrans, O.Trans), (O.Trans, O.NoTrans)])
+def test_gemm_nt_tn_register_path_edges(G, ta, tb):
+    # NoTrans x Trans and Trans x NoTrans with even leading dimensions and K
+    # a multiple of 32: the register-A kernel with B^T staged row by row
+    # (pair loads clamped at an odd N), or with A^T's k runs loaded per row
+    # (rows clamped at M-1); the same edge cases as the NoTrans x NoTrans form
+    for (m, n, k, gap, alpha, beta, nan_c) in (
+            (129, 257, 64, 2, 1.0, 1.0, False), (127, 3, 32, 0, -1.1, 1.3, False),
+            (2113, 201, 96, 4, 1.0, 0.0, True), (3, 131, 32, 1, 0.7, 1.0, False),
+            (1, 1, 32, 2, 1.0, -1.1, False), (384, 385, 512, 0, 1.0, 1.0, False),
+            (255, 129, 160, 2, -1.1, 0.0, True)):
+        r, nbad, mx = gemm_case(G, ta, tb, m, n, k, alpha, beta, gap=gap, nan_c=nan_c,
+                                even=True)
+        assert r < 50.0 and nbad == 0, (m, n, k, gap, alpha, beta, r, nbad, mx)
+
+
+@pytest.mark.parametrize("uplo", [O.Lower, O.Upper])
+@pytest.mark.parametrize("trans", [O.NoTrans, O.Trans])
+def test_syrk_register_path(G, uplo, trans):
+    # SYRK with an even lda and K a multiple of 32 runs the register-A
+    # kernel over the triangle's tiles only; the diagonal tiles split 16-byte
+    # row pairs between kept and untouched elements
+    for (n, k, ldc_gap) in ((257, 64, 2), (129, 32, 1), (600, 96, 0), (3, 32, 1), (385, 160, 3)):
+        na, ka = (n, k) if trans == O.NoTrans else (k, n)
+        a = O.gegen(na + (na & 1), ka, n * k + 3)  # lda even
+        c0 = O.gegen(n, n, n + 17, lda=n + ldc_gap)
+        for alpha, beta in ((1.0, 1.0), (-1.1, 0.0), (0.7, 1.3)):
+            ct = c0.copy(order="F")
+            G.cblas_dsyrk(G.ColMajor, uplo, trans, n, k, alpha, a, a.shape[0], beta, ct,
+                          n + ldc_gap)
+            cr = c0.copy(order="F")
+            O.refsyrk(uplo, trans, n, k, alpha, np.asfortranarray(a[:na]), beta, cr)
+            tri = np.tril if uplo == O.Lower else np.triu
+            other = np.triu if uplo == O.Lower else np.tril
+            off = 1 if uplo == O.Lower else -1
+            assert np.array_equal(other(ct[:n], off), other(c0[:n], off))
+            assert O.gechkgap(ct, n) == 0
+            err = np.abs(tri(ct[:n]) - tri(cr[:n])).max()
+            bound = 2 * (abs(alpha) * 2 * k * 2.0 ** -52 + abs(beta) * 2.0 ** -52) + 2.0 ** -52
+            assert err <= bound, (n, k, alpha, beta, err)
 
 
 def test_gemm_config1_1000(G):
