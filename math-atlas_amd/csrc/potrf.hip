@@ -36,6 +36,19 @@ __device__ __forceinline__ double lane_value(double v, int src) {
                               static_cast<unsigned int>(lo));
 }
 
+// 1/sqrt(d) for d > 0: the hardware estimate and two Newton steps
+// (quadratic convergence from its ~2^-23 relative error to below an ulp);
+// values below 2^-1000, where the estimate loses range, take sqrt and a
+// division.
+__device__ __forceinline__ double rsqrt_pos(double d) {
+  if (d < 0x1p-1000) return 1.0 / sqrt(d);
+  double y = __builtin_amdgcn_rsq(d);
+  const double h = 0.5 * d;
+  y = y * (1.5 - h * y * y);
+  y = y * (1.5 - h * y * y);
+  return y;
+}
+
 constexpr int SB = 32;  // sub-block order of the in-workgroup factorization
 #ifndef ATLG_POTRF_RB
 #define ATLG_POTRF_RB 384
@@ -48,22 +61,26 @@ constexpr long RB = ATLG_POTRF_RB;  // block width of the right-looking outer st
 // triangle, handled as L = U^T, so one algorithm serves both.  The other
 // triangle of A is never read or written.
 //
-// Factorization (right-looking, 32-wide sub-blocks, the block held in LDS;
-// order beyond n is padded with the identity):
-//   1. wave 0 factors the 32 x 32 diagonal sub-block in registers, lane i
-//      holding row i; the column of each step reaches the other lanes by
-//      v_readlane, so no barrier is needed inside the sub-block (the
-//      reciprocal multiply of ATLAS's potrfL closed forms);
-//   2. the rows below solve X L11^T = A21 (one row per thread, L11 read
-//      from LDS);
-//   3. the trailing lower triangle is updated A22 -= X X^T (4 x 4 register
-//      tiles).
+// Factorization (right-looking, 32-wide panels, the block held in LDS;
+// order beyond n is padded with the identity), per panel:
+//   1. wave 0 holds the panel's 32 x 32 diagonal block and the 32 rows
+//      below it in registers, lane i one row, and factors them column by
+//      column: the pivot from lane j by v_readlane, multipliers scaled by
+//      the reciprocal square root (ATLAS's potrfL closed forms multiply by
+//      the reciprocal too), the column's diagonal-block part broadcast to
+//      the other lanes through LDS; no barrier inside the panel;
+//   2. the remaining rows solve X L11^T = A21 right-looking (one row per
+//      thread, the 31 - k updates of step k independent of each other);
+//   3. the trailing lower triangle A22 -= X X^T on the matrix cores.
 // The factor is then inverted in LDS (lower_inverse_inplace).
 __global__ void __launch_bounds__(P_THREADS)
 potf2_inv_kernel(int n, double* __restrict__ A, long lda, int lower, int* __restrict__ dinfo,
                  int info_off, double* __restrict__ Dinv) {
   if (*dinfo != 0) return;
   __shared__ double S[DB * LDS_LD];
+  __shared__ double LT[SB * SB];   // the panel's diagonal block, transposed
+  __shared__ double rinv_sh[SB];   // its reciprocal pivots
+  __shared__ double cbuf[64];      // wave 0's column broadcast
   __shared__ int fail_sh;
   const int t = threadIdx.x;
   // fill in batches of 16 elements per thread, all loads of a batch in
@@ -94,85 +111,122 @@ potf2_inv_kernel(int n, double* __restrict__ A, long lda, int lower, int* __rest
 
   int fail = -1;
   for (int kb = 0; kb < DB; kb += SB) {
-    // 1. diagonal sub-block, wave 0
+    // 1. wave 0: lane i holds panel row kb + i (the 32 x 32 diagonal block
+    //    for i < 32, the next 32 rows below it for i >= 32) and factors the
+    //    panel right-looking: per column j the pivot comes from lane j
+    //    (v_readlane), the multipliers L(i, j) are scaled by the reciprocal
+    //    square root, and column j's part inside the diagonal block reaches
+    //    the other lanes through a wave-private LDS line (ordered within the
+    //    wave: no barrier)
     if (t < 64) {
-      const int i = t;  // lanes 32..63 carry zeros
+      const int i = t, r = kb + i;
+      const bool have = r < DB;
       double x[SB];
 #pragma unroll
       for (int c = 0; c < SB; c++)
-        x[c] = (i < SB && c <= i) ? S[(kb + i) * LDS_LD + kb + c] : 0.0;
-      int f = -1;  // uniform: every lane reads the same pivots
+        x[c] = (have && (i >= SB || c <= i)) ? S[r * LDS_LD + kb + c] : 0.0;
+      // The chain from one pivot to the next is kept short: the next
+      // column's update (one FMA with the multiplier of row j+1, taken by
+      // v_readlane) and the next pivot's reciprocal square root come first;
+      // the other columns' updates read the column from LDS off that path.
+      int f = -1;  // first non-positive pivot (uniform)
+      double d = lane_value(x[0], 0);
+      int bad = __builtin_amdgcn_readfirstlane((int)!(d > 0.0));
+      double y = bad ? 0.0 : rsqrt_pos(d);
+      double yi = 0.0;  // lane i < 32: its own reciprocal pivot
 #pragma unroll
       for (int j = 0; j < SB; j++) {
-        const double d = lane_value(x[j], j);
-        if (f < 0 && !(d > 0.0)) f = j;
-        if (f < 0) {
-          const double l = sqrt(d), rinv = 1.0 / l;
-          x[j] = (i == j) ? l : ((i > j) ? x[j] * rinv : x[j]);
-#pragma unroll
-          for (int c = j + 1; c < SB; c++) {
-            const double lc = lane_value(x[j], c);  // L[c][j]
-            x[c] = (i >= c) ? x[c] - x[j] * lc : x[c];
-          }
+        if (bad) {
+          f = j;
+          break;
         }
+        const double L = (i == j) ? d * y : x[j] * y;
+        if (i >= j) x[j] = L;
+        if (i == j) yi = y;
+        const double Lm = (i > j) ? L : 0.0;  // lanes <= j: no update
+        // column j to LDS and back (all reads in flight together), the
+        // next pivot computed while they travel
+        cbuf[i] = L;
+        double cv[SB];
+#pragma unroll
+        for (int c = j + 2; c < SB; c++) cv[c] = cbuf[c];
+        __builtin_amdgcn_sched_barrier(0);
+        if (j + 1 < SB) {
+          x[j + 1] -= Lm * lane_value(L, j + 1);
+          d = lane_value(x[j + 1], j + 1);
+          bad = __builtin_amdgcn_readfirstlane((int)!(d > 0.0));
+          y = bad ? 0.0 : rsqrt_pos(d);
+        }
+        __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+        for (int c = j + 2; c < SB; c++) x[c] -= Lm * cv[c];
       }
-      if (i < SB) {
+      if (have) {
 #pragma unroll
         for (int c = 0; c < SB; c++)
-          if (c <= i) S[(kb + i) * LDS_LD + kb + c] = x[c];
+          if (i >= SB || c <= i) S[r * LDS_LD + kb + c] = x[c];
       }
-      if (t == 0 && f >= 0) fail_sh = kb + f;
+      if (i < SB) {  // the diagonal block transposed: LT[k][c] = L(kb+c, kb+k)
+        rinv_sh[i] = yi;
+#pragma unroll
+        for (int k = 0; k < SB; k++) LT[k * SB + i] = x[k];
+      }
+      if (t == 0) fail_sh = f >= 0 ? kb + f : -1;
     }
     __syncthreads();
     fail = fail_sh;
     if (fail >= 0) break;
     const int r0 = kb + SB, nrow = DB - r0;
     if (nrow <= 0) break;
-    // 2. rows below: X[i][:] L11^T = A21[i][:]
-    if (t < nrow) {
-      const int i = r0 + t;
+    // 2. the rows beyond wave 0's (kb + 64 ..): X L11^T = A21 right-looking,
+    //    the column updates of one step independent of each other
+    if (t >= 64 && kb + t < DB) {
+      const int r = kb + t;
       double x[SB];
 #pragma unroll
-      for (int c = 0; c < SB; c++) x[c] = S[i * LDS_LD + kb + c];
+      for (int c = 0; c < SB; c++) x[c] = S[r * LDS_LD + kb + c];
 #pragma unroll
-      for (int c = 0; c < SB; c++) {
-        double acc = x[c];
+      for (int k = 0; k < SB; k++) {
+        double cv[SB];  // row k of LT, all reads in flight together
+        const double rk = rinv_sh[k];
 #pragma unroll
-        for (int k = 0; k < c; k++) acc -= x[k] * S[(kb + c) * LDS_LD + kb + k];
-        x[c] = acc / S[(kb + c) * LDS_LD + kb + c];
+        for (int c = k + 1; c < SB; c++) cv[c] = LT[k * SB + c];
+        __builtin_amdgcn_sched_barrier(0);
+        x[k] *= rk;
+#pragma unroll
+        for (int c = k + 1; c < SB; c++) x[c] -= x[k] * cv[c];
       }
 #pragma unroll
-      for (int c = 0; c < SB; c++) S[i * LDS_LD + kb + c] = x[c];
+      for (int c = 0; c < SB; c++) S[r * LDS_LD + kb + c] = x[c];
     }
     __syncthreads();
-    // 3. trailing update of the lower triangle, 4 x 4 tiles
-    const int tiles = nrow / 4, ntile = tiles * (tiles + 1) / 2;
-    for (int u = t; u < ntile; u += P_THREADS) {
-      int ti = (int)((sqrtf(8.0f * u + 1.0f) - 1.0f) * 0.5f);
-      while (ti * (ti + 1) / 2 > u) ti--;
-      while ((ti + 1) * (ti + 2) / 2 <= u) ti++;
-      const int tj = u - ti * (ti + 1) / 2;
-      double acc[4][4] = {};
-#pragma unroll 8
-      for (int c = 0; c < SB; c++) {
-        double xi[4], xj[4];
+    // 3. trailing lower triangle A22 -= X X^T on the matrix cores: 16 x 16
+    //    tiles (ti >= tj) of the nrow x nrow block, K = 32 in eight steps
+    {
+      typedef TriMfma<double> TM;
+      const int lane = t & 63, wv = t >> 6, lx = lane & 15, lk = lane >> 4;
+      const int T = nrow / 16, ntile = T * (T + 1) / 2;
+      for (int u = wv; u < ntile; u += P_THREADS / 64) {
+        int ti = (int)((sqrtf(8.0f * u + 1.0f) - 1.0f) * 0.5f);
+        while (ti * (ti + 1) / 2 > u) ti--;
+        while ((ti + 1) * (ti + 2) / 2 <= u) ti++;
+        const int tj = u - ti * (ti + 1) / 2;
+        const int ra = r0 + 16 * ti + lx, rb = r0 + 16 * tj + lx;
+        double av[SB / 4], bv[SB / 4];
 #pragma unroll
-        for (int a = 0; a < 4; a++) {
-          xi[a] = S[(r0 + 4 * ti + a) * LDS_LD + kb + c];
-          xj[a] = S[(r0 + 4 * tj + a) * LDS_LD + kb + c];
+        for (int q = 0; q < SB / 4; q++) {
+          av[q] = S[ra * LDS_LD + kb + 4 * q + lk];
+          bv[q] = S[rb * LDS_LD + kb + 4 * q + lk];
         }
+        TM::acc_t acc = {0, 0, 0, 0};
 #pragma unroll
-        for (int a = 0; a < 4; a++)
+        for (int q = 0; q < SB / 4; q++) acc = TM::op(av[q], bv[q], acc);
 #pragma unroll
-          for (int b = 0; b < 4; b++) acc[a][b] += xi[a] * xj[b];
+        for (int e = 0; e < 4; e++) {
+          const int gi = r0 + 16 * ti + TM::row(lane, e), gk = r0 + 16 * tj + lx;
+          if (gi >= gk) S[gi * LDS_LD + gk] -= acc[e];
+        }
       }
-#pragma unroll
-      for (int a = 0; a < 4; a++)
-#pragma unroll
-        for (int b = 0; b < 4; b++) {
-          const int i = r0 + 4 * ti + a, k = r0 + 4 * tj + b;
-          if (i >= k) S[i * LDS_LD + k] -= acc[a][b];
-        }
     }
     __syncthreads();
   }

@@ -54,9 +54,16 @@ __device__ inline void tri_level_mfma(R* S, int sz, int n) {
     const int o = q * sz;
     if (tile < nt && o + h < n) {
       const int ar = o + h + 16 * ti + lx, bc = o + 16 * tj + lx;
-      for (int k0 = 16 * tj; k0 < h; k0 += 4) {
-        const int k = o + k0 + lk;
-        acc[u] = M::op(S[ar * LD + k], S[k * LD + bc], acc[u]);
+      for (int k0 = 16 * tj; k0 < h; k0 += 16) {  // four steps' operands in flight
+        R av[4], bv[4];
+#pragma unroll
+        for (int q = 0; q < 4; q++) {
+          const int k = o + k0 + 4 * q + lk;
+          av[q] = S[ar * LD + k];
+          bv[q] = S[k * LD + bc];
+        }
+#pragma unroll
+        for (int q = 0; q < 4; q++) acc[u] = M::op(av[q], bv[q], acc[u]);
       }
     }
   }
@@ -81,9 +88,16 @@ __device__ inline void tri_level_mfma(R* S, int sz, int n) {
     const int o = q * sz;
     if (tile < nt && o + h < n) {
       const int ar = o + h + 16 * ti + lx, bc = o + 16 * tj + lx;
-      for (int k0 = 0; k0 < 16 * (ti + 1); k0 += 4) {
-        const int k = o + h + k0 + lk;
-        acc[u] = M::op(S[ar * LD + k], S[k * LD + bc], acc[u]);
+      for (int k0 = 0; k0 < 16 * (ti + 1); k0 += 16) {
+        R av[4], bv[4];
+#pragma unroll
+        for (int q = 0; q < 4; q++) {
+          const int k = o + h + k0 + 4 * q + lk;
+          av[q] = S[ar * LD + k];
+          bv[q] = S[k * LD + bc];
+        }
+#pragma unroll
+        for (int q = 0; q < 4; q++) acc[u] = M::op(av[q], bv[q], acc[u]);
       }
     }
   }
