@@ -303,12 +303,19 @@ void potrf_rec(hipStream_t s, bool lower, long N, double* A, long lda, int* dinf
 #endif
 constexpr long kCholLaMin = 2048;
 
-// Look-ahead step width: the chain of diagonal blocks (three 128-block
-// leaves and a panel solve per 384 columns) is the longer path at small
-// orders, the trailing SYRK at large ones.  Measured (ms): N=4096 256: 7.9,
-// 384: 8.4; N=8192 128: 15.7, 256: 16.9, 384: 18.1; N=12288 256: 30.7,
-// 384: 31.3; N=16384 256: 55, 384: 52.
-inline long chol_step(long N) { return N < 10240 ? 128 : (N < 14336 ? 256 : 384); }
+// Look-ahead step width (ATLG_CHOL_STEP when set, a multiple of 128): the
+// chain of diagonal blocks (three 128-block leaves and a panel solve per 384
+// columns) is the longer path at small orders, the trailing SYRK at large
+// ones.  Measured on MI355X (ms, panels on 32 compute units): N=8192 128:
+// 11.4, 256: 12.6, 512: 15.2; N=12288 256: 22.9; N=16384 256: 41.0, 384:
+// 39.8, 512: 43.7; 48 or 64 panel units were slower at every order.
+#ifndef ATLG_CHOL_STEP
+#define ATLG_CHOL_STEP 0
+#endif
+inline long chol_step(long N) {
+  if (ATLG_CHOL_STEP > 0) return ATLG_CHOL_STEP;
+  return N < 10240 ? 128 : (N < 14336 ? 256 : 384);
+}
 
 // Right-looking Cholesky by chol_step(N)-column blocks with one step of
 // look-ahead:
