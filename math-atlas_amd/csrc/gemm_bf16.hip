@@ -29,6 +29,12 @@
 
 #include "atlg_internal.hpp"
 
+// No implicit contraction anywhere in this file: the split terms and the
+// epilogue round exactly as written, whatever the compiler's defaults, so the
+// factor (and whether refinement converges on it) does not depend on the
+// build.
+#pragma clang fp contract(off)
+
 namespace atlg {
 namespace {
 
@@ -247,9 +253,12 @@ gemm_bf16_kernel(int M, int N, int Kp, float alpha, const __bf16* __restrict__ A
         }
       }
       f4 v;
+      // rounding pinned: beta * c rounded, then one fused multiply-add
+      // (the file is compiled with contraction off)
 #pragma unroll
       for (int r = 0; r < 4; r++)
-        v[r] = (beta == 0.0f) ? alpha * acc[i][j][r] : alpha * acc[i][j][r] + beta * cin[i][j][r];
+        v[r] = (beta == 0.0f) ? alpha * acc[i][j][r]
+                              : __builtin_fmaf(alpha, acc[i][j][r], beta * cin[i][j][r]);
       if (VEC && gm + 3 < M && gn < N) {
         *reinterpret_cast<f4*>(c) = v;
       } else if (gn < N) {

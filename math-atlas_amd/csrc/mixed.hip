@@ -171,6 +171,7 @@ matvec_finish_kernel(long N, int nchunk, const double* __restrict__ partial,
 // part[chunk][j] = sum over this chunk's rows of V[r][j] * w[r], j < k;
 // V has leading dimension ldv (k <= kMaxBasis + 1)
 constexpr int kMaxBasis = 30;
+constexpr int kInnerBudget = 10 * kMaxBasis;  // preconditioned products per solve, at most
 __global__ void __launch_bounds__(kRows)
 dots_kernel(long N, int k, const double* __restrict__ V, long ldv, const double* __restrict__ w,
             double* __restrict__ part) {
@@ -502,12 +503,14 @@ int dsgesv(hipStream_t s, long N, long NRHS, double* A, long lda, int* ipiv, con
         break;
       }
       if (k == itermax) break;
-      // GMRES-IR gives up early (same ITER code as running out of
-      // iterations) when two corrections in a row have not halved the
-      // residual ratio: each costs up to kMaxBasis preconditioned products
+      // GMRES-IR is restarted GMRES(kMaxBasis) whose restarts take a fresh
+      // FP64 residual: it continues while the corrections make progress and
+      // gives up early (same ITER code as running out of iterations) when
+      // two corrections in a row have cut the residual ratio by less than
+      // 10%, or after kInnerBudget preconditioned products in all
       if (refine) {
-        slow = (ratio > 0.5 * prev_ratio) ? slow + 1 : 0;
-        if (slow >= 2) break;
+        slow = (ratio > 0.9 * prev_ratio) ? slow + 1 : 0;
+        if (slow >= 2 || inner >= kInnerBudget) break;
       }
       prev_ratio = ratio;
       if (refine) {

@@ -311,3 +311,43 @@ def test_dsgesv_gmres_ir(G, lowprec, n, nrhs, kappa):
     # kappa = 1e2 for larger n) and the two-term split one at kappa = 1e2
     if kappa <= (1e6 if lowprec in (0, 2) else 1e2):
         assert it >= 0
+
+
+@pytest.mark.parametrize("lowprec", [0, 2])
+def test_dsgesv_gmres_ir_n16384_kappa1e6(G, lowprec):
+    """GMRES-IR at n = 16384, kappa = 1e6 with the FP32-accurate factors
+    (FP32, split BF16): it must converge without the FP64 fallback and meet
+    the backward-error bound ||b - A x|| / (||A|| ||x|| + ||b||) <= 1e-15 n.
+    A = U diag(s) V^T is generated on the GPU (Haar U, V; s geometric from
+    1 to 1e-6); the residual is an FP64 torch product on the device."""
+    import torch
+    n = 16384
+    g = torch.Generator(device="cuda")
+    g.manual_seed(16384 + lowprec)
+
+    def haar():
+        q, r = torch.linalg.qr(torch.randn((n, n), generator=g, dtype=torch.float64,
+                                           device="cuda"))
+        return q * torch.sign(torch.diagonal(r)).unsqueeze(0)
+
+    U, V = haar(), haar()
+    s = torch.pow(torch.tensor(1e6, dtype=torch.float64, device="cuda"),
+                  -torch.arange(n, dtype=torch.float64, device="cuda") / (n - 1))
+    At = torch.mm(V * s.unsqueeze(0), U.t().contiguous())  # row-major A^T = column-major A
+    del U, V
+    A = At.clone()
+    b = torch.randn(n, generator=g, dtype=torch.float64, device="cuda")
+    x = torch.zeros(n, dtype=torch.float64, device="cuda")
+    ipiv = torch.zeros(n, dtype=torch.int32, device="cuda")
+    info = torch.zeros(1, dtype=torch.int32, device="cuda")
+    it, inner = G.dsgesv(n, 1, A, n, ipiv, b, n, x, n, info, 30, lowprec, refine=1, inner=True)
+    torch.cuda.synchronize()
+    r = b - torch.mv(At.t(), x)
+    anrm = float(At.abs().sum(dim=0).max())
+    be = float(r.abs().max()) / (anrm * float(x.abs().max()) + float(b.abs().max()))
+    print("gmres-ir n=16384 kappa=1e6 lowprec=%d: ITER=%d inner=%d backward error %.3g"
+          % (lowprec, it, inner, be))
+    assert int(info.item()) == 0
+    assert it >= 0, (it, inner)
+    assert be <= 1e-15 * n
+    assert torch.equal(A, At)  # converged: A untouched
