@@ -242,11 +242,13 @@ int atlg_dsposv_batched(int nsys, const int *n, double *const *A,
                         const int *ldb, double *const *X, const int *ldx,
                         int nrhs, int itermax, int *info, int *iter,
                         atlg_stream_t stream);
-/* Decision table of the batched solvers (prec 0 = FP64, 1 = FP32): systems
- * with n <= t16_max use 16-wide block columns, wider ones 32; wg_per_cu
- * persistent workgroups per compute unit.  The defaults come from the
- * empirical search's generated header; set overrides them for this
- * process (the search itself times candidates through it). */
+/* Tuning of the batched solvers (prec 0 = FP64, 1 = FP32): with t16_max
+ * >= 0, systems with n <= t16_max use 16-wide block columns and wider ones
+ * 32; with t16_max < 0 (the default) the decision tree generated by the
+ * empirical search picks the width per system; wg_per_cu persistent
+ * workgroups per compute unit.  The defaults come from the search's
+ * generated header; set overrides them for this process (the search itself
+ * times candidates through it). */
 int atlg_batched_tune_get(int prec, int *t16_max, int *wg_per_cu);
 int atlg_batched_tune_set(int prec, int t16_max, int wg_per_cu);
 

@@ -9,9 +9,9 @@
 //                                              Lbar -= tril(ybar a^T)
 //   potrf:   L = chol(K)                    -> Kbar = sym(L^-T Phi(L^T Lbar) L^-1)
 // where Phi takes the lower triangle with the diagonal halved and
-// sym(X) = (X + X^T)/2 (blocked form of Murray, "Differentiation of the
-// Cholesky decomposition", 2016).  Every product runs on the MFMA GEMM and
-// TRSM of this library.
+// sym(X) = (X + X^T)/2 (Murray, "Differentiation of the Cholesky
+// decomposition", 2016, its blocked matrix form).  Every product runs on the
+// MFMA TRMM and TRSM of this library (3 n^3 flops for the potrf adjoint).
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -84,21 +84,29 @@ unsigned grid_for(long total) {
 
 }  // namespace
 
+// Kbar = sym(L^-T Phi(L^T Lbar) L^-1): P = L^T tril(Lbar) by TRMM, Phi(P),
+// then the two triangular solves with L's diagonal-block inverses computed
+// once, 3 n^3 flops in all (the GEMM form of L^T Lbar took 2 n^3 for that
+// step alone).  A solve that keeps Phi(P) L^-1 lower triangular saves half
+// of one solve's flops, but its recursion's many small launches made it
+// slower at n = 8192 (16 against 10.5 ms).  Only L's lower triangle is read.
 void potrf_adjoint(hipStream_t s, long N, const double* L, long ldl, double* G, long ldg) {
   if (N <= 0) return;
-  double* W = static_cast<double*>(scratch(s, kSlotAdjoint, sizeof(double) * 2 * N * N));
-  double* Lc = W;          // tril(L)
-  double* P = W + N * N;   // Phi(L^T Lbar), then the adjoint
+  constexpr long NB = kTrsmNB;
+  const long nblk = (N + NB - 1) / NB;
+  double* W = static_cast<double*>(
+      scratch(s, kSlotAdjoint, sizeof(double) * ((size_t)N * N + (size_t)nblk * NB * NB)));
+  double* P = W;                       // N x N, ld N
+  double* D = W + (size_t)N * N;       // inv(L_bb)
   const unsigned g = grid_for(N * N);
-  hipLaunchKernelGGL(tril_copy_kernel, dim3(g), dim3(256), 0, s, N, L, ldl, Lc, N);
+  hipLaunchKernelGGL(tril_copy_kernel, dim3(g), dim3(256), 0, s, N, G, ldg, P, N);
   ATLG_LAUNCHED();
-  hipLaunchKernelGGL(tril_copy_kernel, dim3(g), dim3(256), 0, s, N, G, ldg, G, ldg);
-  ATLG_LAUNCHED();
-  gemm(s, kTrans, kNoTrans, N, N, N, 1.0, Lc, N, G, ldg, 0.0, P, N);
+  trmm(s, kLeft, kLower, kTrans, kNonUnit, N, N, 1.0, L, ldl, P, N);
   hipLaunchKernelGGL(phi_kernel, dim3(g), dim3(256), 0, s, N, P, N);
   ATLG_LAUNCHED();
-  trsm(s, kLeft, kLower, kTrans, kNonUnit, N, N, 1.0, L, ldl, P, N);
-  trsm(s, kRight, kLower, kNoTrans, kNonUnit, N, N, 1.0, L, ldl, P, N);
+  tri_inverse_blocks<double>(s, N, L, ldl, kNoTrans, kLower, kNonUnit, D);
+  trsm(s, kLeft, kLower, kTrans, kNonUnit, N, N, 1.0, L, ldl, P, N, nullptr, D, 1);
+  trsm(s, kRight, kLower, kNoTrans, kNonUnit, N, N, 1.0, L, ldl, P, N, nullptr, D, 0);
   hipLaunchKernelGGL(sym_kernel, dim3(g), dim3(256), 0, s, N, P, N, G, ldg);
   ATLG_LAUNCHED();
 }

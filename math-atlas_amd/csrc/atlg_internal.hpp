@@ -224,7 +224,8 @@ void logdet_chol_adjoint(hipStream_t s, long N, const double* L, long ldl, const
 // factor.  Bsrc (may be null) is copied into B (same ldb) before the solve.
 // `orig` (may be null) maps list position -> index into info.
 struct BatchTune {
-  int t16_max;    // systems with n <= t16_max use 16-wide block columns, 32 otherwise
+  int t16_max;    // >= 0: systems with n <= t16_max use 16-wide block columns, 32 otherwise;
+                  // < 0: the generated decision tree chooses per system
   int wg_per_cu;  // persistent workgroups per compute unit
 };
 BatchTune batch_tune(bool single);

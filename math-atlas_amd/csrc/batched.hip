@@ -13,7 +13,7 @@
 // One workgroup (kBW waves) owns a system from start to finish.
 //
 // Per system, left-looking by block columns of width T (16 or 32, chosen per
-// system from the tuned decision table):
+// system on the host by the decision tree the empirical search generates):
 //   U: C = A[J:n, J:J+T] - L[J:n, 0:J] L[J:J+T, 0:J]^T   (MFMA 16x16x4 tiles,
 //      operands straight from global memory, lower part written back);
 //   F: one wave factors the T x T diagonal block in LDS and forms its inverse;
@@ -394,14 +394,15 @@ __device__ void solve_sys(int n, const S* __restrict__ L, int lda, S* __restrict
   }
 }
 
+// width: the system's block-column width (16 or 32), chosen on the host
 template <class S>
-__device__ int chol_any(int n, S* A, int lda, int t16_max, Smem& sm) {
-  return (n <= t16_max) ? chol_sys<S, 16>(n, A, lda, sm) : chol_sys<S, 32>(n, A, lda, sm);
+__device__ int chol_any(int n, S* A, int lda, int width, Smem& sm) {
+  return (width == 16) ? chol_sys<S, 16>(n, A, lda, sm) : chol_sys<S, 32>(n, A, lda, sm);
 }
 template <class S>
-__device__ void solve_any(int n, const S* L, int lda, S* B, int ldb, int nrhs, int t16_max,
+__device__ void solve_any(int n, const S* L, int lda, S* B, int ldb, int nrhs, int width,
                           Smem& sm) {
-  if (n <= t16_max) solve_sys<S, 16>(n, L, lda, B, ldb, nrhs, sm);
+  if (width == 16) solve_sys<S, 16>(n, L, lda, B, ldb, nrhs, sm);
   else solve_sys<S, 32>(n, L, lda, B, ldb, nrhs, sm);
 }
 
@@ -421,14 +422,14 @@ struct SysDesc {
   S* A;
   S* B;
   const S* Bsrc;  // copied into B first when not null
-  int n, lda, ldb, idx;
+  int n, lda, ldb, idx, width;
 };
 
 // mode: 0 factor, 1 factor + solve, 2 solve with a given factor
 template <class S>
 __global__ void __launch_bounds__(kBT)
 posv_batched_kernel(int nsys, const SysDesc<S>* __restrict__ desc, int nrhs, int mode,
-                    int t16_max, int* __restrict__ info, unsigned* __restrict__ counter) {
+                    int* __restrict__ info, unsigned* __restrict__ counter) {
   __shared__ Smem sm;
   for (;;) {
     const int q = next_system(counter, nsys, sm);
@@ -441,8 +442,8 @@ posv_batched_kernel(int nsys, const SysDesc<S>* __restrict__ desc, int nrhs, int
       __syncthreads();
     }
     int f = 0;
-    if (mode != 2) f = chol_any<S>(d.n, d.A, d.lda, t16_max, sm);
-    if (f == 0 && mode != 0) solve_any<S>(d.n, d.A, d.lda, d.B, d.ldb, nrhs, t16_max, sm);
+    if (mode != 2) f = chol_any<S>(d.n, d.A, d.lda, d.width, sm);
+    if (f == 0 && mode != 0) solve_any<S>(d.n, d.A, d.lda, d.B, d.ldb, nrhs, d.width, sm);
     if (threadIdx.x == 0 && mode != 2) info[d.idx] = f;
   }
 }
@@ -454,7 +455,7 @@ struct MixDesc {
   double* X;        // FP64 solution
   float* A32;       // workspace n x n (ld n)
   float* W32;       // workspace n x nrhs (ld n)
-  int n, lda, ldb, ldx, idx, pad;
+  int n, lda, ldb, ldx, idx, width;
 };
 
 __device__ double block_max(double v, Smem& sm) {
@@ -515,7 +516,7 @@ __device__ void residual(const MixDesc& d, int nrhs, Smem& sm, bool* conv, doubl
 }
 
 __global__ void __launch_bounds__(kBT)
-mixed_posv_kernel(int nsys, const MixDesc* __restrict__ desc, int nrhs, int itermax, int t16_max,
+mixed_posv_kernel(int nsys, const MixDesc* __restrict__ desc, int nrhs, int itermax,
                   int* __restrict__ info, int* __restrict__ iter, unsigned* __restrict__ counter) {
   __shared__ Smem sm;
   for (;;) {
@@ -569,7 +570,7 @@ mixed_posv_kernel(int nsys, const MixDesc* __restrict__ desc, int nrhs, int iter
       it = 0;
     } else if (ov != 0.0) {
       it = -2;
-    } else if (chol_any<float>(n, d.A32, n, t16_max, sm) != 0) {
+    } else if (chol_any<float>(n, d.A32, n, d.width, sm) != 0) {
       it = -3;
     } else {
       // X = float solve of B
@@ -577,7 +578,7 @@ mixed_posv_kernel(int nsys, const MixDesc* __restrict__ desc, int nrhs, int iter
         for (int i = threadIdx.x; i < n; i += kBT)
           d.W32[i + (size_t)r * n] = (float)d.B[i + (size_t)r * d.ldb];
       __syncthreads();
-      solve_any<float>(n, d.A32, n, d.W32, n, nrhs, t16_max, sm);
+      solve_any<float>(n, d.A32, n, d.W32, n, nrhs, d.width, sm);
       for (int r = 0; r < nrhs; r++)
         for (int i = threadIdx.x; i < n; i += kBT)
           d.X[i + (size_t)r * d.ldx] = (double)d.W32[i + (size_t)r * n];
@@ -592,7 +593,7 @@ mixed_posv_kernel(int nsys, const MixDesc* __restrict__ desc, int nrhs, int iter
           break;
         }
         if (k == itermax) break;
-        solve_any<float>(n, d.A32, n, d.W32, n, nrhs, t16_max, sm);
+        solve_any<float>(n, d.A32, n, d.W32, n, nrhs, d.width, sm);
         for (int r = 0; r < nrhs; r++)
           for (int i = threadIdx.x; i < n; i += kBT)
             d.X[i + (size_t)r * d.ldx] += (double)d.W32[i + (size_t)r * n];
@@ -661,8 +662,15 @@ unsigned* reset_counter(hipStream_t s) {
   return counter;
 }
 
-BatchTune g_tune[2] = {ATLG_BATCH_TUNE_D, ATLG_BATCH_TUNE_S};
+BatchTune g_tune[2] = {{-1, ATLG_BATCH_WPC_D}, {-1, ATLG_BATCH_WPC_S}};
 std::mutex g_tune_mu;
+
+// Block-column width of one system: the override threshold when set, else
+// the generated decision tree of the empirical search.
+int width_for(const BatchTune& tune, bool single, int n) {
+  if (tune.t16_max >= 0) return n <= tune.t16_max ? 16 : 32;
+  return single ? atlg_batched_width_s(n) : atlg_batched_width_d(n);
+}
 
 }  // namespace
 
@@ -685,13 +693,14 @@ void posv_batched(hipStream_t s, int mode, int nsys, const int* n, S* const* A, 
   for (int q = 0; q < nsys; q++) {
     const int k = ord[q];
     desc[q] = SysDesc<S>{A[k], B ? B[k] : nullptr, Bsrc ? Bsrc[k] : nullptr, n[k], lda[k],
-                         ldb ? ldb[k] : 1, orig ? orig[k] : k};
+                         ldb ? ldb[k] : 1, orig ? orig[k] : k,
+                         width_for(tune, std::is_same<S, float>::value, n[k])};
   }
   auto* dd = static_cast<const SysDesc<S>*>(
       stage_upload(s, desc.data(), sizeof(SysDesc<S>) * nsys, kSlotBatchDesc));
   unsigned* counter = reset_counter(s);
   hipLaunchKernelGGL(posv_batched_kernel<S>, dim3(grid_for(s, tune, nsys)), dim3(kBT), 0, s, nsys,
-                     dd, nrhs, mode, tune.t16_max, info, counter);
+                     dd, nrhs, mode, info, counter);
   ATLG_LAUNCHED();
 }
 
@@ -720,13 +729,13 @@ void dsposv_batched(hipStream_t s, int nsys, const int* n, double* const* A, con
     const int k = ord[q];
     const size_t nn = (size_t)n[k];
     desc[q] = MixDesc{A[k], B[k], X[k], W + off[q], W + off[q] + nn * nn, n[k], lda[k], ldb[k],
-                      ldx[k], k, 0};
+                      ldx[k], k, width_for(tune, true, n[k])};
   }
   auto* dd = static_cast<const MixDesc*>(
       stage_upload(s, desc.data(), sizeof(MixDesc) * nsys, kSlotBatchDesc));
   unsigned* counter = reset_counter(s);
   hipLaunchKernelGGL(mixed_posv_kernel, dim3(grid_for(s, tune, nsys)), dim3(kBT), 0, s, nsys, dd,
-                     nrhs, itermax, tune.t16_max, info, iter, counter);
+                     nrhs, itermax, info, iter, counter);
   ATLG_LAUNCHED();
   // Systems whose refinement did not converge (or whose FP32 factor failed)
   // get the FP64 factor and solve, as LAPACK's dsposv does; A then holds the

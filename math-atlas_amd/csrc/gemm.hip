@@ -576,6 +576,87 @@ __global__ void syreflect_kernel(int N, double* C, long ldc) {
   }
 }
 
+// In-place products with one short side, the diagonal-block applications
+// of TRSM and TRMM: C = alpha op(A) B + beta C with C == B and M, K <= 128
+// (LEFT), or C = alpha A op(B) + beta C with C == A and N, K <= 128.  The
+// 128x128 tiling would give them one workgroup per 128 columns (rows); here
+// a workgroup owns a 32-wide strip of the whole short side instead (4x the
+// workgroups), reads all of its strip (and op(A) / op(B), shared through L2)
+// before a barrier, and only then writes the strip back.  16x16 output tiles,
+// four per wave: LEFT, row tiles 2w, 2w+1 by both column tiles; else both
+// row tiles by column tiles 2w, 2w+1.  T: the short operand transposed.
+constexpr int kStrip = 32;
+template <class S, bool LEFT, bool T>
+__global__ void __launch_bounds__(256)
+gemm_strip(int M, int N, int K, S alpha, const S* __restrict__ A, long lda,
+           const S* __restrict__ B, long ldb, S beta, S* C, long ldc,
+           const int* __restrict__ skip) {
+  typedef Mfma<S> MF;
+  if (skip && *skip) return;
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6, fx = lane & 15, fk = lane >> 4;
+  const int r0 = LEFT ? 0 : (int)blockIdx.x * kStrip;  // first row / column of the strip
+  const int c0 = LEFT ? (int)blockIdx.x * kStrip : 0;
+  int rt[2], ct[2];  // this wave's row / column tile origins
+#pragma unroll
+  for (int e = 0; e < 2; e++) {
+    rt[e] = LEFT ? r0 + 16 * (2 * w + e) : r0 + 16 * e;
+    ct[e] = LEFT ? c0 + 16 * e : c0 + 16 * (2 * w + e);
+  }
+  // element (m, k) of op(A) and (k, n) of op(B); zero outside the matrices
+  auto aval = [&](int m, int k) -> S {
+    if (m >= M || k >= K) return S(0);
+    return (LEFT && T) ? A[k + (size_t)m * lda] : A[m + (size_t)k * lda];
+  };
+  auto bval = [&](int k, int n) -> S {
+    if (k >= K || n >= N) return S(0);
+    return (!LEFT && T) ? B[n + (size_t)k * ldb] : B[k + (size_t)n * ldb];
+  };
+  typename MF::acc_t acc[2][2];
+#pragma unroll
+  for (int i = 0; i < 2; i++)
+#pragma unroll
+    for (int j = 0; j < 2; j++) acc[i][j] = typename MF::acc_t{0, 0, 0, 0};
+  for (int k0 = 0; k0 < K; k0 += 16) {  // four MFMA steps' operands in flight
+    S av[4][2], bv[4][2];
+#pragma unroll
+    for (int q = 0; q < 4; q++)
+#pragma unroll
+      for (int e = 0; e < 2; e++) {
+        av[q][e] = aval(rt[e] + fx, k0 + 4 * q + fk);
+        bv[q][e] = bval(k0 + 4 * q + fk, ct[e] + fx);
+      }
+#pragma unroll
+    for (int q = 0; q < 4; q++)
+#pragma unroll
+      for (int i = 0; i < 2; i++)
+#pragma unroll
+        for (int j = 0; j < 2; j++) acc[i][j] = MF::op(bv[q][j], av[q][i], acc[i][j]);
+  }
+  // lane l, register r: C[rt + (l & 15)][ct + c_n(l, r)]
+  S old[2][2][4];
+#pragma unroll
+  for (int i = 0; i < 2; i++)
+#pragma unroll
+    for (int j = 0; j < 2; j++)
+#pragma unroll
+      for (int r = 0; r < 4; r++) {
+        const int gi = rt[i] + fx, gj = ct[j] + MF::c_n(lane, r);
+        old[i][j][r] = (beta != S(0) && gi < M && gj < N) ? C[gi + (size_t)gj * ldc] : S(0);
+      }
+  __syncthreads();  // every read of the strip precedes every write
+#pragma unroll
+  for (int i = 0; i < 2; i++)
+#pragma unroll
+    for (int j = 0; j < 2; j++)
+#pragma unroll
+      for (int r = 0; r < 4; r++) {
+        const int gi = rt[i] + fx, gj = ct[j] + MF::c_n(lane, r);
+        if (gi < M && gj < N)
+          C[gi + (size_t)gj * ldc] =
+              (beta == S(0)) ? alpha * acc[i][j][r] : alpha * acc[i][j][r] + beta * old[i][j][r];
+      }
+}
+
 template <class S, bool AT, bool BT>
 void launch_t(hipStream_t s, dim3 g, bool va, bool vb, int M, int N, int K, S alpha, const S* A,
               long lda, const S* B, long ldb, S beta, S* C, long ldc, int mode, const int* skip) {
@@ -626,6 +707,30 @@ void gemm(hipStream_t s, int TA, int TB, long M, long N, long K, S alpha, const 
   if (max_wg > 0 && nwg > max_wg) nwg = std::max(8L, (long)(max_wg / 8) * 8);
   const dim3 g((unsigned)nwg);
   const bool at = is_trans(TA), bt = is_trans(TB);
+  // in-place diagonal-block applications (C is B, or C is A) with a short
+  // side of at most 128: strips of 32 across the long side
+  if (mode == 0 && C == B && C != A && M <= BM && K <= BM && !bt) {
+    const dim3 gs((unsigned)((N + kStrip - 1) / kStrip));
+    if (at)
+      hipLaunchKernelGGL((gemm_strip<S, true, true>), gs, dim3(256), 0, s, (int)M, (int)N, (int)K,
+                         alpha, A, lda, B, ldb, beta, C, ldc, skip);
+    else
+      hipLaunchKernelGGL((gemm_strip<S, true, false>), gs, dim3(256), 0, s, (int)M, (int)N,
+                         (int)K, alpha, A, lda, B, ldb, beta, C, ldc, skip);
+    ATLG_LAUNCHED();
+    return;
+  }
+  if (mode == 0 && C == A && C != B && N <= BN && K <= BN && !at) {
+    const dim3 gs((unsigned)((M + kStrip - 1) / kStrip));
+    if (bt)
+      hipLaunchKernelGGL((gemm_strip<S, false, true>), gs, dim3(256), 0, s, (int)M, (int)N,
+                         (int)K, alpha, A, lda, B, ldb, beta, C, ldc, skip);
+    else
+      hipLaunchKernelGGL((gemm_strip<S, false, false>), gs, dim3(256), 0, s, (int)M, (int)N,
+                         (int)K, alpha, A, lda, B, ldb, beta, C, ldc, skip);
+    ATLG_LAUNCHED();
+    return;
+  }
   const bool va = aligned16<S>(A, lda), vb = aligned16<S>(B, ldb);
   // FP64 with aligned operands, K a multiple of 32 and C not one of the
   // operands (TRSM's in-place diagonal apply): A straight to registers, B

@@ -251,7 +251,7 @@ __global__ void sv_reset(int* flags, int n) {
     flags[e] = 0;
 }
 
-template <class S, int R>
+template <class S, int R, bool TR>
 __global__ void __launch_bounds__(SV_T)
 trsv_sf_kernel(int n, int nb, int upper, S alpha, const S* __restrict__ A, long lda,
                const S* __restrict__ Dinv, S* __restrict__ B, long ldb, int* __restrict__ flags,
@@ -271,14 +271,21 @@ trsv_sf_kernel(int n, int nb, int upper, S alpha, const S* __restrict__ A, long 
       const int q = upper ? nb - 1 - lj : lj;
       const long c0 = (long)q * NB;
       const int cols = (int)min((long)NB, n - c0);
-      // this thread's 64 entries of A(block p, block q): issued before the
-      // wait, they do not depend on x_q
+      // this thread's 64 entries of op(A)(block p, block q): issued before
+      // the wait, they do not depend on x_q (TR: op(A) = A^T, the thread's
+      // entries are contiguous in A)
       S va[64];
       {
         const int cend = (r < rows) ? min(64, cols - h * 64) : 0;
-        const S* a = A + (r0 + r) + (size_t)(c0 + h * 64) * lda;
+        if (TR) {
+          const S* a = A + (c0 + h * 64) + (size_t)(r0 + r) * lda;
 #pragma unroll
-        for (int c = 0; c < 64; c++) va[c] = (c < cend) ? a[(size_t)c * lda] : S(0);
+          for (int c = 0; c < 64; c++) va[c] = (c < cend) ? a[c] : S(0);
+        } else {
+          const S* a = A + (r0 + r) + (size_t)(c0 + h * 64) * lda;
+#pragma unroll
+          for (int c = 0; c < 64; c++) va[c] = (c < cend) ? a[(size_t)c * lda] : S(0);
+        }
       }
       if (t == 0) {
         // one round trip per poll; the error word only every 256 polls
@@ -347,10 +354,12 @@ trsv_sf_kernel(int n, int nb, int upper, S alpha, const S* __restrict__ A, long 
   }
 }
 
-// Few-RHS Left/NoTrans solve; returns false when the shape is not handled.
+// Few-RHS Left solve with op(A) = A (tr false) or A^T (tr true), `lower`
+// saying whether op(A) is lower triangular, and Dinv holding the inverses
+// of op(A)'s diagonal blocks; returns false when the shape is not handled.
 template <class S>
-bool trsv_few(hipStream_t s, bool lower, long n, long nrhs, S alpha, const S* A, long lda,
-              const S* Dinv, S* B, long ldb) {
+bool trsv_few(hipStream_t s, bool lower, bool tr, long n, long nrhs, S alpha, const S* A,
+              long lda, const S* Dinv, S* B, long ldb) {
   if (nrhs < 1 || nrhs > 4) return false;
   const int nb = (int)((n + NB - 1) / NB);
   int* flags = static_cast<int*>(scratch(s, kSlotTrsv, sizeof(int) * (nb + 64)));
@@ -367,11 +376,15 @@ bool trsv_few(hipStream_t s, bool lower, long n, long nrhs, S alpha, const S* A,
   long limit = wait_limit(kSvSpin);
   void* args[] = {&nn, &nbk, &up, &alpha, &A, &lda, &Dinv, &B, &ldb, &flags, &err, &status, &limit};
   const void* fn = nullptr;
-  switch (nrhs) {
-    case 1: fn = reinterpret_cast<const void*>(&trsv_sf_kernel<S, 1>); break;
-    case 2: fn = reinterpret_cast<const void*>(&trsv_sf_kernel<S, 2>); break;
-    case 3: fn = reinterpret_cast<const void*>(&trsv_sf_kernel<S, 3>); break;
-    default: fn = reinterpret_cast<const void*>(&trsv_sf_kernel<S, 4>); break;
+  switch (nrhs + (tr ? 4 : 0)) {
+    case 1: fn = reinterpret_cast<const void*>(&trsv_sf_kernel<S, 1, false>); break;
+    case 2: fn = reinterpret_cast<const void*>(&trsv_sf_kernel<S, 2, false>); break;
+    case 3: fn = reinterpret_cast<const void*>(&trsv_sf_kernel<S, 3, false>); break;
+    case 4: fn = reinterpret_cast<const void*>(&trsv_sf_kernel<S, 4, false>); break;
+    case 5: fn = reinterpret_cast<const void*>(&trsv_sf_kernel<S, 1, true>); break;
+    case 6: fn = reinterpret_cast<const void*>(&trsv_sf_kernel<S, 2, true>); break;
+    case 7: fn = reinterpret_cast<const void*>(&trsv_sf_kernel<S, 3, true>); break;
+    default: fn = reinterpret_cast<const void*>(&trsv_sf_kernel<S, 4, true>); break;
   }
   ATLG_HIP(hipLaunchCooperativeKernel(fn, dim3(G), dim3(SV_T), args, 0, s));
   return true;
@@ -422,8 +435,8 @@ void trsm(hipStream_t s, int side, int uplo, int ta, int diag, long M, long N, S
   c.dtrans = dinv ? (dinv_trans ? kTrans : kNoTrans) : kNoTrans;
   if (dinv) {
     c.Dinv = dinv;
-    if (c.left && !c.trans && !dinv_trans && !skip && na >= kTrsvMin &&
-        trsv_few<S>(s, uplo == kLower, na, N, alpha, A, lda, dinv, B, ldb))
+    if (c.left && !dinv_trans && !skip && na >= kTrsvMin &&
+        trsv_few<S>(s, c.lower_eff, c.trans, na, N, alpha, A, lda, dinv, B, ldb))
       return;
   } else {
     const long nblk = (na + NB - 1) / NB;
@@ -431,8 +444,8 @@ void trsm(hipStream_t s, int side, int uplo, int ta, int diag, long M, long N, S
     tri_inverse_blocks<S>(s, na, A, lda, ta, uplo, diag, D, skip);
     c.Dinv = D;
     // a few right-hand sides on a long triangle: one sync-free launch
-    if (c.left && !c.trans && !skip && na >= kTrsvMin &&
-        trsv_few<S>(s, uplo == kLower, na, N, alpha, A, lda, D, B, ldb))
+    if (c.left && !skip && na >= kTrsvMin &&
+        trsv_few<S>(s, c.lower_eff, c.trans, na, N, alpha, A, lda, D, B, ldb))
       return;
   }
   solve(c, 0, na, alpha);

@@ -5,19 +5,28 @@ search of AtlasBase/Clint/atlas-tlp.base:5578-6220, GetMultSampleTimes /
 findNB, and the generated-header emitters such as atlas-mmcpy.base:3369):
 
 * every candidate is timed several times and the minimum is kept;
-* each timing is appended to a results file, and a later run reads the file
-  and times only what is missing, so an interrupted search resumes;
+* each timing is appended as one ATLAS-style summary line
+  ("ID=7 ROUT=dposv_batched T=16 WPC=5 N=256 NSYS=512 NRHS=4 TIME=... MFLOP=...",
+  MFLOP being the rate in MFLOP/s, as in ATLAS's *.sum files) to
+  tune/res/<rout>.sum; a later run reads the file and times only what is
+  missing, so an interrupted search resumes;
 * the search is coarse first, then refined: block-column width per order
   (16 versus 32 on a ladder of orders, then bisection between the last order
   where 16 wins and the first where 32 wins), then workgroups per compute
   unit on a sample of the uneven workload (a coarse ladder, then the
   neighbours of the best);
-* the winners are written as a C header, math-atlas_amd/csrc/
-  atlg_batched_tune.h, that the library compiles in as its defaults.
+* the header math-atlas_amd/csrc/atlg_batched_tune.h, which the library
+  compiles in as its defaults, is generated from the *.sum files alone: per
+  precision the workgroups per compute unit of the fastest uneven-sample
+  line, and a decision tree on the system order (ATLAS's generated
+  decision code of atlas-tlp.base:5126-5300 decides on problem size the
+  same way) whose leaves are the fastest block-column width on each
+  interval between measured orders.
 
 Usage (on a machine with the GPU, the library built):
     python math-atlas_amd/tune/batched_search.py [--prec d|s|both]
         [--res DIR] [--out HEADER] [--samples K] [--quick]
+    python math-atlas_amd/tune/batched_search.py --emit-only   (no GPU: header from *.sum)
 """
 import argparse
 import os
@@ -28,33 +37,57 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 PKG = os.path.dirname(HERE)
 sys.path.insert(0, PKG)
 
-import numpy as np  # noqa: E402
-import torch  # noqa: E402
-
-import atlas_gpu as G  # noqa: E402
-
 BIG = 1 << 30  # t16_max that forces 16-wide block columns for every order
+ROUT = {"d": "dposv_batched", "s": "sposv_batched"}
 
 
-class Results:
-    """Append-only timing cache: one 'key value' line per measurement."""
+def flops_posv(n, nrhs):
+    """Cholesky plus two triangular solves: n^3/3 + 2 n^2 nrhs."""
+    return n ** 3 / 3.0 + 2.0 * n * n * nrhs
 
-    def __init__(self, path):
-        self.path = path
-        self.t = {}
+
+class Sum:
+    """Append-only summary file, one 'KEY=value ...' line per timing."""
+
+    KEYS = ("T", "WPC", "N", "NSYS", "NRHS")
+
+    def __init__(self, path, rout):
+        self.path, self.rout, self.lines = path, rout, []
         if os.path.exists(path):
             for line in open(path):
-                parts = line.split()
-                if len(parts) == 2:
-                    self.t[parts[0]] = float(parts[1])
+                d = dict(tok.split("=", 1) for tok in line.split() if "=" in tok)
+                if d.get("ROUT") == rout and "TIME" in d:
+                    self.lines.append(d)
 
-    def get(self, key, fn):
-        if key not in self.t:
-            v = fn()
-            self.t[key] = v
+    @staticmethod
+    def key(d):
+        return tuple(str(d[k]) for k in Sum.KEYS)
+
+    def find(self, **kw):
+        k = self.key({a: str(b) for a, b in kw.items()})
+        for d in self.lines:
+            if self.key(d) == k:
+                return float(d["TIME"])
+        return None
+
+    def get(self, fn, mflop, **kw):
+        t = self.find(**kw)
+        if t is None:
+            t = fn()
+            d = {k: str(v) for k, v in kw.items()}
+            d.update(ID=str(len(self.lines) + 1), ROUT=self.rout, TIME="%.6e" % t,
+                     MFLOP="%.1f" % (mflop / t))
+            self.lines.append(d)
             with open(self.path, "a") as f:
-                f.write("%s %.6e\n" % (key, v))
-        return self.t[key]
+                f.write(" ".join("%s=%s" % (k, d[k]) for k in
+                                 ("ID", "ROUT") + self.KEYS + ("TIME", "MFLOP")) + "\n")
+        return t
+
+
+def _gpu():
+    import torch
+    import atlas_gpu as G
+    return torch, G
 
 
 class Workload:
@@ -62,6 +95,7 @@ class Workload:
     with nrhs normal right-hand sides; restored before every timing."""
 
     def __init__(self, sizes, nrhs, dtype, seed):
+        torch, _ = _gpu()
         g = torch.Generator(device="cuda").manual_seed(seed)
         self.n = [int(x) for x in sizes]
         self.dtype = dtype
@@ -91,8 +125,10 @@ class Workload:
         self.pa = [self.A.data_ptr() + o * es for o in self.aoff]
         self.pb = [self.B.data_ptr() + o * es for o in boff]
         self.info = torch.zeros(len(self.n), dtype=torch.int32, device="cuda")
+        self.mflop = sum(flops_posv(n, nrhs) for n in self.n) / 1e6
 
     def run(self):
+        torch, G = _gpu()
         self.A.copy_(self.A0)
         self.B.copy_(self.B0)
         fn = G.dposv_batched if self.dtype == torch.float64 else G.sposv_batched
@@ -108,17 +144,23 @@ class Workload:
 
 
 def min_time(work, prec, t16, wpc, samples):
+    _, G = _gpu()
     G.set_batched_tune(prec, t16, wpc)
     work.run()  # warm-up
     return min(work.run() for _ in range(samples))
 
 
 def search(prec, res, samples, quick):
+    """Times the candidates missing from `res` (a Sum); returns nothing:
+    the header is emitted from the summary files afterwards."""
+    import numpy as np
+    torch, G = _gpu()
     dtype = torch.float64 if prec == 0 else torch.float32
     tag = "d" if prec == 0 else "s"
     cus = torch.cuda.get_device_properties(0).multi_processor_count
-    _, wpc0 = G.batched_tune(prec)
-    nsys_uniform = 2 * cus  # one full wave of persistent workgroups, twice
+    _, wpc0 = G.batched_tune(prec)  # the compiled-in default (nothing was set yet)
+    nsys = 2 * cus  # one full wave of persistent workgroups, twice
+    nrhs = 4
 
     # -- stage 1: block-column width per order (coarse ladder) -------------
     ladder = [16, 32, 48, 64, 96, 128, 192, 256, 384, 512]
@@ -126,82 +168,97 @@ def search(prec, res, samples, quick):
         ladder = [32, 64, 128, 256, 512]
 
     def t_at(n, width):
-        key = "%s_width%d_n%d_w%d" % (tag, width, n, wpc0)
-        work = None
-
         def measure():
-            nonlocal work
-            work = Workload([n] * nsys_uniform, 4, dtype, seed=n)
+            work = Workload([n] * nsys, nrhs, dtype, seed=n)
             return min_time(work, prec, BIG if width == 16 else 0, wpc0, samples)
-
-        return res.get(key, measure)
+        return res.get(measure, nsys * flops_posv(n, nrhs) / 1e6, T=width, WPC=wpc0, N=n,
+                       NSYS=nsys, NRHS=nrhs)
 
     wins16 = {n: t_at(n, 16) <= t_at(n, 32) for n in ladder}
     print("[%s] width 16 wins at:" % tag, [n for n in ladder if wins16[n]], flush=True)
-    # crossover: last ladder order where 16 wins before 32 wins for good
-    last16 = 0
-    for n in ladder:
-        if wins16[n]:
-            last16 = n
-    # -- refine: bisection between last16 and the next ladder order -------
-    above = [n for n in ladder if n > last16]
-    if last16 and above and not quick:
-        lo, hi = last16, above[0]
-        while hi - lo > 8:
-            mid = (lo + hi) // 2
-            if t_at(mid, 16) <= t_at(mid, 32):
-                lo = mid
-            else:
-                hi = mid
-        last16 = lo
-    t16_max = last16
-    print("[%s] t16_max = %d" % (tag, t16_max), flush=True)
+    # refine every change of winner between neighbouring ladder orders by
+    # bisection (to 8), so the decision tree's boundaries are measured
+    if not quick:
+        for a, b in zip(ladder, ladder[1:]):
+            if wins16[a] == wins16[b]:
+                continue
+            lo, hi = a, b
+            while hi - lo > 8:
+                mid = (lo + hi) // 2
+                if (t_at(mid, 16) <= t_at(mid, 32)) == wins16[a]:
+                    lo = mid
+                else:
+                    hi = mid
 
     # -- stage 2: workgroups per CU on the uneven workload sample ----------
     rng = np.random.default_rng(2024)
     sample = rng.integers(16, 513, size=(600 if quick else 2000)).tolist()
-    work = Workload(sample, 4, dtype, seed=7)
+    work = Workload(sample, nrhs, dtype, seed=7)
 
     def t_wpc(w):
-        key = "%s_uneven%d_t%d_w%d" % (tag, len(sample), t16_max, w)
-        return res.get(key, lambda: min_time(work, prec, t16_max, w, samples))
+        return res.get(lambda: min_time(work, prec, -1, w, samples), work.mflop, T="tree", WPC=w,
+                       N="u%d" % len(sample), NSYS=len(sample), NRHS=nrhs)
 
     coarse = [1, 2, 4, 8]
     best = min(coarse, key=t_wpc)
     for w in (best - 1, best + 1):
         if 1 <= w <= 16:
             t_wpc(w)
-    cand = [w for w in range(1, 17)
-            if ("%s_uneven%d_t%d_w%d" % (tag, len(sample), t16_max, w)) in res.t]
-    best = min(cand, key=t_wpc)
-    print("[%s] wg_per_cu = %d (%.3f ms on the %d-system sample)"
-          % (tag, best, 1e3 * t_wpc(best), len(sample)), flush=True)
-    table = [(n, t_at(n, 16), t_at(n, 32)) for n in ladder]
-    return t16_max, best, table, [(w, t_wpc(w)) for w in sorted(cand)]
+    G.set_batched_tune(prec, -1, wpc0)
 
 
-def emit(path, found, device):
+def decide(lines):
+    """From one precision's summary lines: (workgroups per CU, intervals),
+    intervals a list of (last order, width) covering every order."""
+    occ = [(float(d["TIME"]), int(d["WPC"])) for d in lines if d["N"].startswith("u")]
+    wpc = min(occ)[1] if occ else 2
+    by_n = {}
+    for d in lines:
+        if d["N"].startswith("u"):
+            continue
+        by_n.setdefault(int(d["N"]), {})[int(d["T"])] = float(d["TIME"])
+    pts = sorted((n, min(t, key=t.get)) for n, t in by_n.items() if len(t) == 2)
+    if not pts:
+        return wpc, [(1 << 30, 16)]
+    iv = []
+    for (n0, w0), (n1, w1) in zip(pts, pts[1:]):
+        if w0 != w1:
+            iv.append(((n0 + n1) // 2, w0))  # boundary halfway between measured orders
+    iv.append((1 << 30, pts[-1][1]))
+    return wpc, iv
+
+
+def tree(iv, lo, hi, ind):
+    """Balanced if/else over intervals iv[lo:hi] (C source lines)."""
+    if hi - lo == 1:
+        return ["%sreturn %d;" % (ind, iv[lo][1])]
+    mid = (lo + hi) // 2
+    return (["%sif (n <= %d) {" % (ind, iv[mid - 1][0])] + tree(iv, lo, mid, ind + "  ") +
+            ["%s} else {" % ind] + tree(iv, mid, hi, ind + "  ") + ["%s}" % ind])
+
+
+def emit(path, res_dir, device=""):
+    """Writes the header from the summary files in res_dir."""
     lines = [
-        "// Decision table of the batched SPD solvers: {t16_max, wg_per_cu} for FP64",
-        "// (D) and FP32 (S).  Generated by math-atlas_amd/tune/batched_search.py on",
-        "// %s, %s." % (device, time.strftime("%Y-%m-%d")),
-        "//",
+        "// Defaults of the batched SPD solvers, generated by",
+        "// math-atlas_amd/tune/batched_search.py from its summary files",
+        "// (math-atlas_amd/tune/res/*.sum)%s." % ((" measured on " + device) if device else ""),
+        "// Per precision: persistent workgroups per compute unit, and the",
+        "// block-column width (16 or 32) as a decision tree on the system order.",
+        "#pragma once",
     ]
     for tag in ("d", "s"):
-        if tag not in found:
-            continue
-        t16, wpc, table, occ = found[tag]
-        lines.append("// %s: order -> seconds with 16-wide / 32-wide block columns" % tag.upper())
-        for n, a, b in table:
-            lines.append("//   n=%4d  %.4e  %.4e%s" % (n, a, b, "  *16" if a <= b else ""))
-        lines.append("// %s: workgroups per CU -> seconds on the uneven sample" % tag.upper())
-        for w, t in occ:
-            lines.append("//   %2d  %.4e%s" % (w, t, "  *" if w == wpc else ""))
-    lines.append("#pragma once")
-    d = found.get("d", (96, 2))[:2]
-    s = found.get("s", (128, 2))[:2]
-    lines.append("#define ATLG_BATCH_TUNE_D {%d, %d}" % d)
-    lines.append("#define ATLG_BATCH_TUNE_S {%d, %d}" % s)
+        sm = Sum(os.path.join(res_dir, ROUT[tag] + ".sum"), ROUT[tag])
+        wpc, iv = decide(sm.lines)
+        lines.append("//")
+        lines.append("// %s (%d lines):" % (ROUT[tag], len(sm.lines)))
+        for d in sm.lines:
+            lines.append("//   " + " ".join("%s=%s" % (k, d[k]) for k in
+                                            ("T", "WPC", "N", "NSYS", "TIME", "MFLOP")))
+        lines.append("#define ATLG_BATCH_WPC_%s %d" % (tag.upper(), wpc))
+        lines.append("static inline int atlg_batched_width_%s(int n) {" % tag)
+        lines += tree(iv, 0, len(iv), "  ")
+        lines.append("}")
     with open(path, "w") as f:
         f.write("\n".join(lines) + "\n")
     print("wrote", path)
@@ -214,16 +271,20 @@ def main():
     ap.add_argument("--out", default=os.path.join(PKG, "csrc", "atlg_batched_tune.h"))
     ap.add_argument("--samples", type=int, default=3)
     ap.add_argument("--quick", action="store_true")
+    ap.add_argument("--emit-only", action="store_true")
     args = ap.parse_args()
     os.makedirs(args.res, exist_ok=True)
-    assert torch.cuda.is_available()
-    assert G.lib().atlg_init() == 0
-    found = {}
-    for tag, prec in (("d", 0), ("s", 1)):
-        if args.prec in (tag, "both"):
-            res = Results(os.path.join(args.res, "batched_%s.txt" % tag))
-            found[tag] = search(prec, res, args.samples, args.quick)
-    emit(args.out, found, torch.cuda.get_device_name(0))
+    device = ""
+    if not args.emit_only:
+        torch, G = _gpu()
+        assert torch.cuda.is_available()
+        assert G.lib().atlg_init() == 0
+        device = torch.cuda.get_device_name(0)
+        for tag, prec in (("d", 0), ("s", 1)):
+            if args.prec in (tag, "both"):
+                search(prec, Sum(os.path.join(args.res, ROUT[tag] + ".sum"), ROUT[tag]),
+                       args.samples, args.quick)
+    emit(args.out, args.res, device)
 
 
 if __name__ == "__main__":

@@ -272,6 +272,30 @@ def test_trsm_few_rhs_long_triangle(G, uplo, diag):
             assert r < 50.0, (m, n, alpha, r)
 
 
+@pytest.mark.parametrize("uplo", UPLOS)
+@pytest.mark.parametrize("diag", DIAGS)
+def test_trsm_few_rhs_long_triangle_trans(G, uplo, diag):
+    # Left/Trans with 1..4 right-hand sides on triangles >= 1024: the same
+    # sync-free block solve reading A's blocks transposed (each thread's
+    # entries contiguous in A), blocks taken in the opposite order
+    for (m, n) in ((1024, 1), (1500, 2), (2600, 3), (1111, 4)):
+        for alpha in (1.0, -1.1):
+            lda, ldb = m + 2, m + 1
+            a = O.gegen(m, m, m * m + 513 * 3 + 91, lda=lda)
+            O.lib().orc_trddom(uplo, m, O._p(a), lda)
+            if diag == O.Unit:
+                a[np.arange(m), np.arange(m)] = np.nan   # never read
+            b0 = O.gegen(m, n, m * n + 127 * 51 + 77, lda=ldb)
+            bt = b0.copy(order="F")
+            G.cblas_dtrsm(G.ColMajor, O.Left, uplo, O.Trans, diag, m, n, alpha, a, lda, bt, ldb)
+            br = b0.copy(order="F")
+            O.lib().orc_reftrsm(O.Left, uplo, O.Trans, diag, m, n, alpha, O._p(a), lda,
+                                O._p(br), ldb)
+            assert O.gechkgap(bt, m) == 0
+            err = np.abs(bt[:m] - br[:m]).max() / np.abs(br[:m]).max()
+            assert err < 1e-13, (m, n, alpha, err)
+
+
 def test_trsm_rowmajor_and_device_api(G):
     import torch
     m, n = 300, 200
