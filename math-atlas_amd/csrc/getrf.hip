@@ -1054,6 +1054,12 @@ panel_coop(int M, int w, int off, S* __restrict__ P, long lda, int* __restrict__
     const int bw = (p % GT) / CP_THREADS;  // workgroup that published row p
     const S* prow = crows + bw * PW;        // the pivot row
     const S* pmsk = crowm + bw * PW;        // ... with columns <= j and >= w zeroed
+    // the masked pivot row into registers first, every read in flight
+    // together and ahead of the reciprocal (read one by one inside the
+    // update, each FMA waited on its own LDS round trip)
+    S pm[PW];
+#pragma unroll
+    for (int c = 0; c < PW; c++) pm[c] = pmsk[c];
     const S piv = prow[j];
     // the pivots and the first zero pivot stay in LDS until the end: a
     // global store here would stall this wave on its completion (stores
@@ -1089,7 +1095,7 @@ panel_coop(int M, int w, int off, S* __restrict__ P, long lda, int* __restrict__
           else l = x / piv;
           if (isfinite(l)) {
 #pragma unroll
-            for (int c = 0; c < PW; c++) v[q][c] -= l * pmsk[c];
+            for (int c = 0; c < PW; c++) v[q][c] -= l * pm[c];
           } else {
 #pragma unroll
             for (int c = 0; c < PW; c++)
