@@ -24,7 +24,9 @@ import oracle as O
 
 pytestmark = pytest.mark.gpu
 
-EPS64 = np.finfo(np.float64).eps / 2   # ATL_epsilon: 2^-53
+# unit roundoffs 2^-53 and 2^-24: half of ATL_epsilon (2^-52 in double), so the
+# residual bounds below are twice as strict as the testers'
+EPS64 = np.finfo(np.float64).eps / 2
 EPS32 = np.finfo(np.float32).eps / 2
 
 
