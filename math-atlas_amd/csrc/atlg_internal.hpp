@@ -58,10 +58,11 @@ hipStream_t resolve_stream(atlg_stream_t s);
 hipStream_t library_stream();
 
 // A second library stream on the current device (non-blocking, default
-// priority) and two events, for work that overlaps the caller's stream
+// priority) and kAuxEvents events, for work that overlaps the caller's stream
 // (getrf's look-ahead panel).  Created on first use.
 hipStream_t aux_stream();
-hipEvent_t aux_event(int i);  // i = 0..3
+constexpr int kAuxEvents = 6;
+hipEvent_t aux_event(int i);  // i = 0 .. kAuxEvents-1
 // A pair of streams restricted to disjoint compute units: *small gets the
 // first `cus` units, *large the rest (hipExtStreamCreateWithCUMask).  Made
 // once per device and unit count; false when cus <= 0 or the masked streams

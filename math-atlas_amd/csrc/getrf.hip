@@ -43,6 +43,7 @@
 #include <vector>
 
 #include "atlg_internal.hpp"
+#include "tri_inv.hpp"
 
 namespace atlg {
 namespace {
@@ -118,6 +119,15 @@ inline long la_block(long mn) {
   return std::min(1024L, std::max(512L, mn / 32 / 128 * 128));
 }
 constexpr long kLaMin = 2048;
+// Split column of a look-ahead block of width nb (0: not split), as the
+// recursion would choose it.
+#ifndef ATLG_LA_SPLIT
+#define ATLG_LA_SPLIT 1
+#endif
+inline long la_split(long nb) {
+  if (!ATLG_LA_SPLIT || nb < 2 * kTrsmNB) return 0;
+  return ((nb / 2) / kTrsmNB) * kTrsmNB;
+}
 
 // ---------------------------------------------------------------- helpers
 
@@ -1228,6 +1238,115 @@ panel_step(int M, int w, int j, int mn, int off, S* __restrict__ A, long lda,
 
 __global__ void set_int_kernel(int* p, int v) { *p = v; }
 
+// ---------------------------------------------------------------- small nodes
+
+// A recursion node of width K (32 or 64) with nc <= 64 columns to its right,
+// in one launch: U12 = inv(L11) A12 and A22 -= L21 U12, L11 being the node's
+// K x K unit lower block.  Every workgroup inverts L11 in LDS and forms U12
+// itself, so no workgroup waits for another.  The last workgroup to have
+// read A12 (a ticket counted after the reads have returned, reset by that
+// workgroup for the next launch) writes U12 over it.  Each workgroup then updates 64 rows of A22 on the matrix cores (one
+// wave per 16 rows), with U12 from LDS as the first operand so that a lane's
+// results are consecutive rows of one column.  Its rows of L21 and A22 are
+// loaded first and arrive while the inverse is formed.  This is the TRSM and
+// GEMM of getrfC's recursion step (AtlasBase/Clint/atlas-lp.base:1540-1575)
+// for the narrowest nodes, where three launches cost more than the work.
+template <class S, int K>
+__global__ void __launch_bounds__(kTriThreads)
+lu_node_small(int mrows, int nc, const S* __restrict__ L11, S* __restrict__ A12,
+              const S* __restrict__ L21, S* __restrict__ A22, long lda,
+              unsigned* __restrict__ ticket) {
+  static_assert(K == 32 || K == 64, "node width");
+  typedef TriMfma<S> Mf;
+  typedef typename Mf::acc_t acc_t;
+  constexpr int LD = K + 1;   // pitch of the inverse, [row][col]
+  constexpr int UP = 64 + 1;  // pitch of A12 / U12, [k][col]
+  constexpr int KT = K / 16;  // row tiles of U12
+  __shared__ S Li[K * LD];
+  __shared__ S U[K * UP];
+  __shared__ int writer;
+  const int t = threadIdx.x, lane = t & 63, wv = t >> 6, lx = lane & 15, lk = lane >> 4;
+  const int row = blockIdx.x * 64 + 16 * wv + lx;  // row of A22 / L21
+  const bool rv = row < mrows;
+  S l21[K / 4], c[4][4];
+#pragma unroll
+  for (int q = 0; q < K / 4; q++) l21[q] = rv ? L21[row + (size_t)(4 * q + lk) * lda] : S(0);
+#pragma unroll
+  for (int ct = 0; ct < 4; ct++)
+#pragma unroll
+    for (int r = 0; r < 4; r++) {
+      const int col = 16 * ct + Mf::row(lane, r);
+      c[ct][r] = (rv && col < nc) ? A22[row + (size_t)col * lda] : S(0);
+    }
+  {
+    constexpr int PL = K * K / kTriThreads, PU = K * 64 / kTriThreads;
+    S vl[PL], vu[PU];
+#pragma unroll
+    for (int u = 0; u < PL; u++) {
+      const int e = t + u * kTriThreads, i = e % K, j = e / K;
+      vl[u] = i > j ? L11[i + (size_t)j * lda] : (i == j ? S(1) : S(0));
+    }
+#pragma unroll
+    for (int u = 0; u < PU; u++) {
+      const int e = t + u * kTriThreads, i = e % K, j = e / K;
+      vu[u] = j < nc ? A12[i + (size_t)j * lda] : S(0);
+    }
+#pragma unroll
+    for (int u = 0; u < PL; u++) {
+      const int e = t + u * kTriThreads;
+      Li[(e % K) * LD + e / K] = vl[u];
+    }
+#pragma unroll
+    for (int u = 0; u < PU; u++) {
+      const int e = t + u * kTriThreads;
+      U[(e % K) * UP + e / K] = vu[u];
+    }
+  }
+  __syncthreads();  // every read of A12 has returned (its values are in LDS)
+  if (t == 0) {
+    writer = atomicAdd(ticket, 1u) == gridDim.x - 1;
+    if (writer) __hip_atomic_store(ticket, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  }
+  lower_inverse_inplace<S, K>(Li, K);  // ends with a barrier
+  // U12 = inv(L11) A12: wave wv forms column tile wv, every row tile
+  const bool cw = 16 * wv < nc;
+  acc_t u12[KT];
+#pragma unroll
+  for (int rt = 0; rt < KT; rt++) {
+    u12[rt] = acc_t{0, 0, 0, 0};
+    if (cw) {
+      for (int k0 = 0; k0 < 16 * (rt + 1); k0 += 4)  // inv(L11) is lower
+        u12[rt] = Mf::op(Li[(16 * rt + lx) * LD + k0 + lk], U[(k0 + lk) * UP + 16 * wv + lx],
+                         u12[rt]);
+    }
+  }
+  __syncthreads();
+  if (cw) {
+#pragma unroll
+    for (int rt = 0; rt < KT; rt++)
+#pragma unroll
+      for (int r = 0; r < 4; r++) {
+        const int i = 16 * rt + Mf::row(lane, r), col = 16 * wv + lx;
+        U[i * UP + col] = u12[rt][r];
+        if (writer && col < nc) A12[i + (size_t)col * lda] = u12[rt][r];
+      }
+  }
+  __syncthreads();
+  // A22 -= L21 U12 as (U12^T L21^T)^T
+#pragma unroll
+  for (int ct = 0; ct < 4; ct++) {
+    if (16 * ct >= nc) continue;
+    acc_t acc = acc_t{0, 0, 0, 0};
+#pragma unroll
+    for (int q = 0; q < K / 4; q++) acc = Mf::op(U[(4 * q + lk) * UP + 16 * ct + lx], l21[q], acc);
+#pragma unroll
+    for (int r = 0; r < 4; r++) {
+      const int col = 16 * ct + Mf::row(lane, r);
+      if (rv && col < nc) A22[row + (size_t)col * lda] = c[ct][r] - acc[r];
+    }
+  }
+}
+
 // ---------------------------------------------------------------- drivers
 
 template <class S>
@@ -1392,6 +1511,31 @@ void node_trsm(LU<S>& f, long off, long nl, long nr) {
   node_trsm_cols(f, off, nl, off + nl, nr);
 }
 
+// The node at `off` (nl pivot columns, nr columns to their right) in one
+// lu_node_small launch when it is narrow enough; false otherwise.
+#ifndef ATLG_LU_NODE_SMALL
+#define ATLG_LU_NODE_SMALL 1
+#endif
+template <class S>
+bool node_small(LU<S>& f, long off, long nl, long nr) {
+  if (!ATLG_LU_NODE_SMALL || !(nl == 32 || nl == 64) || nr <= 0 || nr > 64) return false;
+  const long mrows = f.M - off - nl;
+  if (mrows >= INT_MAX - 64) return false;
+  const unsigned grid = (unsigned)std::max(1L, (mrows + 63) / 64);
+  const S* L11 = at(f, off, off);
+  S* A12 = at(f, off, off + nl);
+  const S* L21 = at(f, off + nl, off);
+  S* A22 = at(f, off + nl, off + nl);
+  if (nl == 32)
+    hipLaunchKernelGGL((lu_node_small<S, 32>), dim3(grid), dim3(kTriThreads), 0, f.s, (int)mrows,
+                       (int)nr, L11, A12, L21, A22, f.lda, f.ticket);
+  else
+    hipLaunchKernelGGL((lu_node_small<S, 64>), dim3(grid), dim3(kTriThreads), 0, f.s, (int)mrows,
+                       (int)nr, L11, A12, L21, A22, f.lda, f.ticket);
+  ATLG_LAUNCHED();
+  return true;
+}
+
 // Factors rows [off, M) x columns [off, off + n).
 template <class S>
 void getrf_rec(LU<S>& f, long off, long n) {
@@ -1411,9 +1555,11 @@ void getrf_rec(LU<S>& f, long off, long n) {
   else nl = std::max<long>(PW, ((mn / 2) / PW) * PW);
   const long nr = n - nl;
   getrf_rec(f, off, nl);
-  node_trsm(f, off, nl, nr);
-  node_update(f, m - nl, nr, nl, at(f, off + nl, off), at(f, off, off + nl),
-              at(f, off + nl, off + nl));
+  if (!node_small(f, off, nl, nr)) {
+    node_trsm(f, off, nl, nr);
+    node_update(f, m - nl, nr, nl, at(f, off + nl, off), at(f, off, off + nl),
+                at(f, off + nl, off + nl));
+  }
   getrf_rec(f, off + nl, nr);
 }
 
@@ -1561,6 +1707,14 @@ void apply_deferred_left(LU<S>& f, long NB, long mn) {
 //   3. hands block k+1 to the auxiliary stream, which factors it while 2
 //      runs (it waits for 1 only);
 //   4. waits for the panel before step k+1.
+// A block of at least two 128-blocks is factored as the recursion's top node
+// written out: left half, its TRSM and GEMM on the right half, right half,
+// with each half's leaf interchanges confined to the half.  As soon as the
+// left half is done, the caller's stream applies the left half's share of
+// step 1 for the next block (interchanges, TRSM and a GEMM with K = half)
+// while the right half is factored, so that only the right half's share
+// waits for the whole panel; the left half receives the right half's
+// interchanges afterwards, before step 2 reads the block.
 // The columns each stream touches are disjoint while both run.  Partial
 // pivoting picks from the same updated columns as the plain recursion, so
 // the pivots agree wherever they are well separated.
@@ -1599,15 +1753,28 @@ void getrf_lookahead(LU<S>& f0) {
   const long kLaNB = la_block(mn);
   f0.win_hi = std::min(kLaNB, mn);
   getrf_rec(f0, 0, f0.win_hi);
+  hipEvent_t e_left = aux_event(4);
+  long hk = 0;  // split column of block k within it (0: factored whole)
   for (long k0 = 0; k0 < mn; k0 += kLaNB) {
     const long kb = std::min(kLaNB, mn - k0), k1 = k0 + kb;
     const long nb1 = std::min(kLaNB, mn - k1);  // next block (none after the last)
     if (nb1 > 0) {
-      apply_block_lists(f0, k0, kb, k1, nb1, 0, 0);
-      node_trsm_cols(f0, k0, kb, k1, nb1);
-      node_update(f0, M - k1, nb1, kb, at(f0, k1, k0), at(f0, k0, k1), at(f0, k1, k1));
+      if (hk > 0) {
+        // the left half's share of this update ran beside the right half's
+        // panel (below); the right half's remains
+        const long kr = k0 + hk, wr = kb - hk;
+        apply_block_lists(f0, kr, wr, k1, nb1, 0, 0);
+        node_trsm_cols(f0, kr, wr, k1, nb1);
+        node_update(f0, M - k1, nb1, wr, at(f0, k1, kr), at(f0, kr, k1), at(f0, k1, k1));
+      } else {
+        apply_block_lists(f0, k0, kb, k1, nb1, 0, 0);
+        node_trsm_cols(f0, k0, kb, k1, nb1);
+        node_update(f0, M - k1, nb1, kb, at(f0, k1, k0), at(f0, k0, k1), at(f0, k1, k1));
+      }
       ATLG_HIP(hipEventRecord(e_upd, s0));
     }
+    // a split block's left half has not seen its right half's interchanges
+    if (hk > 0) apply_block_lists(f0, k0 + hk, kb - hk, k0, hk, 0, 0);
     // The rest of step k is queued before the panel's many small launches,
     // so s0 has work while the host issues them.
     const long r0 = k1 + nb1, rest = N - r0;
@@ -1620,13 +1787,42 @@ void getrf_lookahead(LU<S>& f0) {
         f0.gemm_cap = 0;
       }
     }
+    hk = 0;
     if (nb1 > 0) {
+      const long h1 = la_split(nb1), k2 = k1 + nb1, nb2 = std::min(kLaNB, mn - k2);
       ATLG_HIP(hipStreamWaitEvent(s1, e_upd, 0));
-      f1.win_lo = k1;
-      f1.win_hi = k1 + nb1;
-      getrf_rec(f1, k1, nb1);
+      if (h1 > 0) {
+        // the recursion's top node by hand, each half's leaves confined to
+        // the half, so that the left half's L columns stay as they are
+        // while the right half is factored
+        f1.win_lo = k1;
+        f1.win_hi = k1 + h1;
+        getrf_rec(f1, k1, h1);
+        ATLG_HIP(hipEventRecord(e_left, s1));
+        apply_block_lists(f1, k1, h1, k1 + h1, nb1 - h1, 0, 0);
+        node_trsm_cols(f1, k1, h1, k1 + h1, nb1 - h1);
+        node_update(f1, M - k1 - h1, nb1 - h1, h1, at(f1, k1 + h1, k1), at(f1, k1, k1 + h1),
+                    at(f1, k1 + h1, k1 + h1));
+        f1.win_lo = k1 + h1;
+        f1.win_hi = k1 + nb1;
+        getrf_rec(f1, k1 + h1, nb1 - h1);
+      } else {
+        f1.win_lo = k1;
+        f1.win_hi = k1 + nb1;
+        getrf_rec(f1, k1, nb1);
+      }
       ATLG_HIP(hipEventRecord(e_panel, s1));
+      if (h1 > 0 && nb2 > 0) {
+        // the left half's share of the next block's update, beside the
+        // right half's panel (s0 has finished the rest of step k)
+        ATLG_HIP(hipStreamWaitEvent(s0, e_left, 0));
+        apply_block_lists(f0, k1, h1, k2, nb2, 0, 0);
+        node_trsm_cols(f0, k1, h1, k2, nb2);
+        node_update(f0, M - k1 - h1, nb2, h1, at(f0, k1 + h1, k1), at(f0, k1, k2),
+                    at(f0, k1 + h1, k2));
+      }
       ATLG_HIP(hipStreamWaitEvent(s0, e_panel, 0));
+      hk = h1;
     }
   }
   apply_deferred_left(f0, kLaNB, mn);

@@ -68,7 +68,7 @@ hipStream_t library_stream() {
 namespace {
 struct Aux {
   hipStream_t stream = nullptr;
-  hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
+  hipEvent_t ev[kAuxEvents] = {};
   // CU-masked pairs by the small side's unit count, created on first use
   // ({nullptr, nullptr}: unavailable)
   std::map<int, std::pair<hipStream_t, hipStream_t>> split;
@@ -92,7 +92,7 @@ std::mutex g_aux_mu;
 std::unique_lock<std::mutex> aux_lock() { return std::unique_lock<std::mutex>(g_aux_mu); }
 
 hipStream_t aux_stream() { return aux_for_device().stream; }
-hipEvent_t aux_event(int i) { return aux_for_device().ev[i & 3]; }
+hipEvent_t aux_event(int i) { return aux_for_device().ev[(unsigned)i % kAuxEvents]; }
 
 bool split_streams(int cus, hipStream_t* small, hipStream_t* large) {
   Aux& a = aux_for_device();
