@@ -122,7 +122,10 @@ int atlg_dpotrs(int Uplo, long N, long NRHS, const double *A, long lda,
  * cblas_dsymm, cblas_dsyr2k, ATL_dtrtri, ATL_dlauum and clapack_dpotri).
  * atlg_dtrtri / atlg_dpotri write info to *dinfo (device): the 1-based
  * index of the first zero on the diagonal of the triangle (A then
- * untouched), else 0. */
+ * untouched), else 0.  From order 256 on, while a dense copy of the
+ * triangle (order rounded up to 32) fits 4 GiB, trtri / lauum / potri work
+ * on such a copy in the library's per-stream scratch (allocated on first
+ * use and kept); larger orders take the in-place recursions. */
 int atlg_dtrmm(int Side, int Uplo, int TA, int Diag, long M, long N,
                double alpha, const double *A, long lda, double *B, long ldb,
                atlg_stream_t stream);

@@ -3,7 +3,8 @@
 Forward and backward both run in libatlas_gpu.so: potrf, the triangular
 solve and logdet forward, and the hand-written adjoint kernels
 (atlg_dpotrf_adjoint, atlg_dtrsm_adjoint, atlg_dlogdet_chol_adjoint)
-backward.  Matrices are torch float64 CUDA tensors in their natural
+backward of the primitives.  The fused log-likelihood's backward uses the
+closed form of its gradient (potri, a triangular solve and a SYRK).  Matrices are torch float64 CUDA tensors in their natural
 orientation (K symmetric, L lower triangular with K = L L^T); the column-major
 buffers the library works on are built internally.
 
@@ -24,8 +25,8 @@ import math
 
 import torch
 
-from . import (Lower, NoTrans, NonUnit, Left, dpotrf, dtrsm, dpotrf_adjoint, dtrsm_adjoint,
-               dlogdet_chol, dlogdet_chol_adjoint)
+from . import (Lower, NoTrans, Trans, NonUnit, Left, dpotrf, dpotri, dsyrk, dtrsm,
+               dpotrf_adjoint, dtrsm_adjoint, dlogdet_chol, dlogdet_chol_adjoint)
 
 
 def _check_info(info, what):
@@ -169,14 +170,21 @@ class GaussianLogLik(torch.autograd.Function):
             X, _ = _rhs_buf(y)
             dtrsm(Left, Lower, NoTrans, NonUnit, n, k, 1.0, buf, n, X, n)
         gval = g.detach().to(torch.float64)
-        # loss = -1/2 |a|^2 - 1/2 logdet  ->  abar = -g a, dbar = -g/2
-        Xbar = -gval * X
-        G = torch.zeros((n, n), dtype=torch.float64, device=K.device)  # Lbar, column-major
-        dtrsm_adjoint(n, k, buf, n, X, n, Xbar, n, G, n)
-        dbar = (-0.5 * k * gval).reshape(1).contiguous()
-        dlogdet_chol_adjoint(n, buf, n, dbar, G, n)
-        dpotrf_adjoint(n, buf, n, G, n)
-        ybar = Xbar if ctx.vec else Xbar.t()
+        # The node's gradient in closed form (Rasmussen & Williams 2006,
+        # eq. 5.9): with A = K^-1 Y = L^-T X,
+        #   dloss/dK = g/2 (A A^T - k K^-1),   dloss/dY = -g A,
+        # K^-1 = L^-T L^-1 by potri (trtri + lauum, 2N^3/3 flops) in place of
+        # the chain of the three adjoints (about 3N^3).  The lower triangle
+        # of K^-1 takes the rank-k term by SYRK, then is mirrored.
+        A = X.clone()
+        dtrsm(Left, Lower, Trans, NonUnit, n, k, 1.0, buf, n, A, n)
+        W = buf.clone() if ctx.store else buf
+        info = torch.zeros(1, dtype=torch.int32, device=W.device)
+        dpotri(Lower, n, W, n, info)  # cannot fail: the factor's diagonal is positive
+        dsyrk(Lower, NoTrans, n, k, 0.5, A, n, -0.5 * k, W, n)
+        # W's column-major lower triangle is the tensor's upper one
+        G = torch.triu(W).add_(torch.triu(W, 1).t()).mul_(gval)
+        ybar = _rhs_out(A.mul_(-gval), y)
         return G, ybar, None
 
 

@@ -99,7 +99,9 @@ enum Slot : int {
   kSlotLaTemp = 18,     // staging of one column block for those interchanges
   kSlotTrmmBlk = 19,    // dense diagonal blocks of op(A) for TRMM leaves
   kSlotSymm = 20,       // dense symmetric copy of A for SYMM
-  kNumSlots = 21
+  kSlotTriFull = 21,    // dense zero-padded copy of a triangle (TRTRI, LAUUM, POTRI)
+  kSlotTriTemp = 22,    // products of the doubling inverse
+  kNumSlots = 23
 };
 void* scratch(hipStream_t s, Slot slot, size_t bytes);
 
@@ -130,6 +132,17 @@ template <class S>
 void gemm(hipStream_t s, int TA, int TB, long M, long N, long K, S alpha, const S* A, long lda,
           const S* B, long ldb, S beta, S* C, long ldc, int mode = 0,
           const int* skip = nullptr, int max_wg = 0);
+// FP64 register-path GEMM over `batch` problems whose operands sit at fixed
+// element strides (sa, sb, sc), optionally restricted per 128x128 tile to
+// the K range a triangular operand with a zero other triangle can reach
+// (ktrim bit 1: B lower, k >= tile column; bit 2: A^T lower, k >= tile row;
+// bit 4: A lower, k < tile row end).  Requires K a multiple of 32, aligned
+// operands, even strides and no aliasing of C with A or B; returns false
+// (launching nothing) when the register path does not apply.
+bool gemm_batched(hipStream_t s, int TA, int TB, long M, long N, long K, double alpha,
+                  const double* A, long lda, long sa, const double* B, long ldb, long sb,
+                  double beta, double* C, long ldc, long sc, int batch, int mode, int ktrim,
+                  const int* skip);
 // C <- beta*C on the M x N (or triangular per mode) part; beta==0 zeroes.
 template <class S>
 void scale(hipStream_t s, long M, long N, S beta, S* C, long ldc, int mode = 0,

@@ -185,6 +185,137 @@ put_tri_blocks_kernel(int na, double* __restrict__ A, long lda, int lower, int u
   }
 }
 
+// ---- dense-scratch forms of TRTRI / LAUUM / POTRI --------------------------
+// The triangle is copied into a dense scratch matrix X (order np = n rounded
+// up to 32, leading dimension np) as a lower triangle with zeros everywhere
+// else (an upper triangle is transposed in), so that the products below can
+// take whole tiles: TRTRI by doubling, LAUUM as one triangle-masked product.
+constexpr long kTriFullMin = 2 * NB;                  // smallest order that uses them
+constexpr size_t kTriFullCap = size_t(4) << 30;       // largest dense copy (bytes)
+constexpr int kCT = 32;                               // copy tile
+
+// X = the lower triangle op(A) (transposed in when !lower; 1 on a unit
+// diagonal), zero elsewhere, in 32x32 tiles through LDS so that both the
+// reads and the writes are coalesced.
+__global__ void __launch_bounds__(256)
+tri_copy_in_kernel(int n, int np, const double* __restrict__ A, long lda, int lower, int unit,
+                   double* __restrict__ X, const int* __restrict__ skip) {
+  if (skip && *skip) return;
+  __shared__ double tl[kCT][kCT + 1];
+  const int tiles = np / kCT;
+  const int i0 = (blockIdx.x % tiles) * kCT, j0 = (blockIdx.x / tiles) * kCT;
+  const int tx = threadIdx.x & 31, ty = threadIdx.x >> 5;  // 32 x 8
+  for (int r = ty; r < kCT; r += 8) {
+    // lower: element (i0+tx, j0+r) = A(i0+tx, j0+r); upper: element
+    // (i0+r, j0+tx) = A(j0+tx, i0+r), read along A's column
+    double v;
+    if (lower) {
+      const int i = i0 + tx, j = j0 + r;
+      v = (i < n && j <= i) ? A[i + (size_t)j * lda] : 0.0;
+      tl[tx][r] = v;
+    } else {
+      const int i = i0 + r, j = j0 + tx;
+      v = (i < n && j <= i) ? A[j + (size_t)i * lda] : 0.0;
+      tl[r][tx] = v;
+    }
+  }
+  __syncthreads();
+  for (int r = ty; r < kCT; r += 8) {
+    const int i = i0 + tx, j = j0 + r;
+    double v = tl[tx][r];
+    if (unit && i == j && i < n) v = 1.0;
+    X[i + (size_t)j * np] = v;
+  }
+}
+
+// The lower triangle of X back into A's triangle (transposed when !lower);
+// a unit diagonal is not written.
+__global__ void __launch_bounds__(256)
+tri_copy_out_kernel(int n, int np, const double* __restrict__ X, int lower, int unit,
+                    double* __restrict__ A, long lda, const int* __restrict__ skip) {
+  if (skip && *skip) return;
+  __shared__ double tl[kCT][kCT + 1];
+  const int tiles = np / kCT;
+  const int i0 = (blockIdx.x % tiles) * kCT, j0 = (blockIdx.x / tiles) * kCT;
+  if (j0 > i0 + kCT - 1 || i0 >= n || j0 >= n) return;  // no lower element in the tile
+  const int tx = threadIdx.x & 31, ty = threadIdx.x >> 5;
+  for (int r = ty; r < kCT; r += 8) tl[tx][r] = X[(i0 + tx) + (size_t)(j0 + r) * np];
+  __syncthreads();
+  for (int r = ty; r < kCT; r += 8) {
+    if (lower) {
+      const int i = i0 + tx, j = j0 + r;
+      if (i < n && j < i + (unit ? 0 : 1)) A[i + (size_t)j * lda] = tl[tx][r];
+    } else {
+      const int i = i0 + r, j = j0 + tx;  // A(j, i) = X(i, j)
+      if (i < n && j < i + (unit ? 0 : 1)) A[j + (size_t)i * lda] = tl[r][tx];
+    }
+  }
+}
+
+long tri_full_order(long n) { return (n + kCT - 1) / kCT * kCT; }
+bool tri_full_fits(long n) {
+  const long np = tri_full_order(n);
+  return n >= kTriFullMin && (size_t)np * np * sizeof(double) <= kTriFullCap;
+}
+
+double* tri_copy_in(hipStream_t s, bool lower, bool unit, long n, const double* A, long lda,
+                    const int* skip) {
+  const long np = tri_full_order(n);
+  double* X = static_cast<double*>(scratch(s, kSlotTriFull, sizeof(double) * np * np));
+  const long tiles = np / kCT;
+  hipLaunchKernelGGL(tri_copy_in_kernel, dim3((unsigned)(tiles * tiles)), dim3(256), 0, s, (int)n,
+                     (int)np, A, lda, lower ? 1 : 0, unit ? 1 : 0, X, skip);
+  ATLG_LAUNCHED();
+  return X;
+}
+
+// X (order n, leading dimension np, lower, zeros elsewhere) <- inv(X):
+// the NB-blocks' inverses, then doubling: for blocks of order sz = 2NB,
+// 4NB, ... with halves already inverted, X21 <- -X22 (X21 X11).  The full
+// blocks of a level go in one batched launch per product, each product
+// restricted to the K range its triangular operand can reach; a ragged
+// last block takes two ordinary products (its zero padding keeps them exact).
+void tri_invert_full(hipStream_t s, int diag, long n, double* X, long np, const int* skip) {
+  const long nblk = (n + NB - 1) / NB;
+  double* D = static_cast<double*>(scratch(s, kSlotTrsmInv, sizeof(double) * nblk * NB * NB));
+  tri_inverse_blocks<double>(s, n, X, np, kNoTrans, kLower, diag, D, skip);
+  hipLaunchKernelGGL(put_tri_blocks_kernel, dim3((unsigned)nblk), dim3(256), 0, s, (int)n, X, np,
+                     1, 0, D, skip);
+  ATLG_LAUNCHED();
+  long tneed = 0;
+  for (long sz = 2 * NB; sz / 2 < n; sz *= 2) tneed = std::max(tneed, std::max(n / sz, 1L) * (sz / 2) * (sz / 2));
+  if (tneed == 0) return;
+  double* T = static_cast<double*>(scratch(s, kSlotTriTemp, sizeof(double) * tneed));
+  for (long sz = 2 * NB; sz / 2 < n; sz *= 2) {
+    const long h = sz / 2, nfull = n / sz, st = sz * (np + 1);
+    if (nfull > 0) {
+      // T_b = X21_b X11_b (X11 lower: k >= tile column), then
+      // X21_b = -X22_b T_b (X22 lower: k < tile row end)
+      if (!gemm_batched(s, kNoTrans, kNoTrans, h, h, h, 1.0, X + h, np, st, X, np, st, 0.0, T, h,
+                        h * h, (int)nfull, 0, 1, skip) ||
+          !gemm_batched(s, kNoTrans, kNoTrans, h, h, h, -1.0, X + h + h * np, np, st, T, h, h * h,
+                        0.0, X + h, np, st, (int)nfull, 0, 4, skip))
+        throw HipFailure{"trtri: the batched product does not apply"};
+    }
+    const long o = nfull * sz, r = n - o - h;
+    if (r > 0) {
+      gemm<double>(s, kNoTrans, kNoTrans, r, h, h, 1.0, X + o + h + o * np, np, X + o + o * np, np,
+                   0.0, T, h, 0, skip);
+      gemm<double>(s, kNoTrans, kNoTrans, r, h, r, -1.0, X + (o + h) + (o + h) * np, np, T, h, 0.0,
+                   X + o + h + o * np, np, 0, skip);
+    }
+  }
+}
+
+// A's triangle <- X^T X (lower: L^T L; upper, X = U^T: U U^T), one product
+// masked to the triangle, each tile's K range starting at its row and column.
+void lauum_from_full(hipStream_t s, bool lower, long n, const double* X, long np, double* A,
+                     long lda, const int* skip) {
+  if (!gemm_batched(s, kTrans, kNoTrans, n, n, np, 1.0, X, np, 0, X, np, 0, 0.0, A, lda, 0, 1,
+                    lower ? 1 : 2, 3, skip))
+    throw HipFailure{"lauum: the masked product does not apply"};
+}
+
 // In-place LAUUM of an n <= NB block: lower: A <- L^T L (lower triangle);
 // upper: A <- U U^T (upper triangle).  The block goes to LDS; each thread
 // computes entries of the result.
@@ -343,6 +474,16 @@ void trtri(hipStream_t s, int uplo, int diag, long N, double* A, long lda, int* 
     hipLaunchKernelGGL(diag_zero_kernel, dim3(1), dim3(256), 0, s, (int)N, A, lda, dinfo);
     ATLG_LAUNCHED();
   }
+  if (tri_full_fits(N)) {
+    const long np = tri_full_order(N);
+    double* X = tri_copy_in(s, lower, diag == kUnit, N, A, lda, dinfo);
+    tri_invert_full(s, diag, N, X, np, dinfo);
+    const long tiles = np / kCT;
+    hipLaunchKernelGGL(tri_copy_out_kernel, dim3((unsigned)(tiles * tiles)), dim3(256), 0, s,
+                       (int)N, (int)np, X, lower ? 1 : 0, diag == kUnit ? 1 : 0, A, lda, dinfo);
+    ATLG_LAUNCHED();
+    return;
+  }
   const long nblk = (N + NB - 1) / NB;
   double* D = static_cast<double*>(scratch(s, kSlotTrsmInv, sizeof(double) * nblk * NB * NB));
   tri_inverse_blocks<double>(s, N, A, lda, kNoTrans, uplo, diag, D, dinfo);
@@ -354,12 +495,30 @@ void trtri(hipStream_t s, int uplo, int diag, long N, double* A, long lda, int* 
 
 void lauum(hipStream_t s, int uplo, long N, double* A, long lda, const int* skip) {
   if (N <= 0) return;
-  lauum_rec(s, uplo == kLower, N, A, lda, skip);
+  const bool lower = uplo == kLower;
+  if (tri_full_fits(N)) {
+    const double* X = tri_copy_in(s, lower, false, N, A, lda, skip);
+    lauum_from_full(s, lower, N, X, tri_full_order(N), A, lda, skip);
+    return;
+  }
+  lauum_rec(s, lower, N, A, lda, skip);
 }
 
 // clapack_dpotri (atlas-clp.base:316-362): trtri, then lauum only when the
 // factor had no zero on its diagonal (the device info word skips it).
 void potri(hipStream_t s, int uplo, long N, double* A, long lda, int* dinfo) {
+  if (N > 0 && tri_full_fits(N)) {
+    // the inverse stays in the dense copy, from which the product reads
+    const bool lower = uplo == kLower;
+    set_int(s, dinfo, 0);
+    hipLaunchKernelGGL(diag_zero_kernel, dim3(1), dim3(256), 0, s, (int)N, A, lda, dinfo);
+    ATLG_LAUNCHED();
+    const long np = tri_full_order(N);
+    double* X = tri_copy_in(s, lower, false, N, A, lda, dinfo);
+    tri_invert_full(s, kNonUnit, N, X, np, dinfo);
+    lauum_from_full(s, lower, N, X, np, A, lda, dinfo);
+    return;
+  }
   trtri(s, uplo, kNonUnit, N, A, lda, dinfo);
   lauum(s, uplo, N, A, lda, dinfo);
 }

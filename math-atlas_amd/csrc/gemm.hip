@@ -320,11 +320,15 @@ gemm_mfma(int M, int N, int K, S alpha, const S* A, long lda, const S* B, long l
 // diagonal (SYRK and the Cholesky update), enumerated as gemm_mfma does.
 constexpr int NN_DA = 2;  // K tiles of A in flight
 constexpr int kBufDword3 = 0x00020000;  // raw buffer, 32-bit dwords (gfx9 family)
-template <bool AT, bool BT>
+// XT: the extended form (a batch over blockIdx.y at element strides sa, sb,
+// sc, and the per-tile K range of ktrim); without it the code is that of the
+// plain product, whose register allocation it would otherwise disturb.
+template <bool AT, bool BT, bool XT>
 __device__ __forceinline__ void gemm_rega_f64(int M, int N, int K, double alpha,
                                               const double* __restrict__ A, long lda,
                                               const double* B, long ldb, double beta, double* C,
-                                              long ldc, int vecc, int mode) {
+                                              long ldc, int vecc, int mode, long sa, long sb,
+                                              long sc, int ktrim) {
   typedef Mfma<double> MF;
   typedef double d2 __attribute__((ext_vector_type(2)));
   constexpr int LDB_S = BN + PADB;
@@ -334,7 +338,11 @@ __device__ __forceinline__ void gemm_rega_f64(int M, int N, int K, double alpha,
   const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
   const int wm0 = (wave >> 1) * WM, wn0 = (wave & 1) * WN;
   const int fx = lane & 15, fk = lane >> 4;
-  const int nk = K / BK;
+  if (XT) {  // batch member blockIdx.y: operands at fixed element strides
+    A += (size_t)blockIdx.y * sa;
+    B += (size_t)blockIdx.y * sb;
+    C += (size_t)blockIdx.y * sc;
+  }
   for (int bid = blockIdx.x; bid < ntiles; bid += gridDim.x) {
     int tm, tn;
     if (mode == 0) {
@@ -347,13 +355,26 @@ __device__ __forceinline__ void gemm_rega_f64(int M, int N, int K, double alpha,
       tri_tile_of(id, mode, tilesM, tilesN, tm, tn);
     }
     const int m0 = tm * BM, n0 = tn * BN;
+    // K range of the tile: all of K, or with a triangular operand whose
+    // other triangle is zero only the part that can be nonzero (bit 1: B
+    // lower, k >= n0; bit 2: A^T lower, k >= m0; bit 4: A lower, k < m0+BM).
+    // The bounds are multiples of 2*BK (m0, n0 and K are); the callers'
+    // shapes keep every range nonempty (m0, n0 < K), which the prologue's
+    // loads assume.
+    int kb = 0, ke = K;
+    if (XT) {
+      if (ktrim & 1) kb = max(kb, n0);
+      if (ktrim & 2) kb = max(kb, m0);
+      if (ktrim & 4) ke = min(ke, m0 + BM);
+    }
+    const int nk = max(0, ke - kb) / BK;
     // Addresses: a wave-uniform base per tile and K step plus a 32-bit
     // per-lane element offset inside the tile.  A: this lane's row pairs,
     // clamped so a pair starts at most at M-1 (lda is even, so the pair
     // stays inside the allocation).  A^T: the lane's four rows (clamped to
     // M-1), each holding k = 4fk .. 4fk+3 contiguously; the MFMA steps then
     // take k = 4fk + ks (any k order serves, as long as B's matches).
-    const double* At = AT ? A + (size_t)m0 * lda : A + m0;
+    const double* At = AT ? A + (size_t)m0 * lda + kb : A + m0 + (size_t)kb * lda;
     unsigned aoff[AT ? 4 : 2];  // bytes
     if (AT) {
 #pragma unroll
@@ -398,7 +419,7 @@ __device__ __forceinline__ void gemm_rega_f64(int M, int N, int K, double alpha,
     // columns per lane, the pair start clamped to at most N-1 (ldb is even,
     // so the pair stays inside the allocation).
     constexpr int B_PER = BN / 8 / (NT / 64);
-    const double* Bt = BT ? B + n0 : B + (size_t)n0 * ldb;
+    const double* Bt = BT ? B + n0 + (size_t)kb * ldb : B + (size_t)n0 * ldb + kb;
     unsigned boff[B_PER];  // bytes
 #pragma unroll
     for (int i = 0; i < B_PER; i++)
@@ -531,7 +552,17 @@ gemm_nn_f64(int M, int N, int K, double alpha, const double* __restrict__ A, lon
             const double* B, long ldb, double beta, double* C, long ldc, int vecc, int mode,
             const int* __restrict__ skip) {
   if (skip && *skip) return;
-  gemm_rega_f64<false, false>(M, N, K, alpha, A, lda, B, ldb, beta, C, ldc, vecc, mode);
+  gemm_rega_f64<false, false, false>(M, N, K, alpha, A, lda, B, ldb, beta, C, ldc, vecc, mode, 0, 0,
+                               0, 0);
+}
+
+__global__ void __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(2, 2)))
+gemm_nn_f64_x(int M, int N, int K, double alpha, const double* __restrict__ A, long lda,
+            const double* B, long ldb, double beta, double* C, long ldc, int vecc, int mode,
+            const int* __restrict__ skip, long sa, long sb, long sc, int ktrim) {
+  if (skip && *skip) return;
+  gemm_rega_f64<false, false, true>(M, N, K, alpha, A, lda, B, ldb, beta, C, ldc, vecc, mode, sa, sb,
+                              sc, ktrim);
 }
 
 __global__ void __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(2, 2)))
@@ -539,7 +570,17 @@ gemm_nt_f64(int M, int N, int K, double alpha, const double* __restrict__ A, lon
             const double* B, long ldb, double beta, double* C, long ldc, int vecc, int mode,
             const int* __restrict__ skip) {
   if (skip && *skip) return;
-  gemm_rega_f64<false, true>(M, N, K, alpha, A, lda, B, ldb, beta, C, ldc, vecc, mode);
+  gemm_rega_f64<false, true, false>(M, N, K, alpha, A, lda, B, ldb, beta, C, ldc, vecc, mode, 0, 0,
+                               0, 0);
+}
+
+__global__ void __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(2, 2)))
+gemm_nt_f64_x(int M, int N, int K, double alpha, const double* __restrict__ A, long lda,
+            const double* B, long ldb, double beta, double* C, long ldc, int vecc, int mode,
+            const int* __restrict__ skip, long sa, long sb, long sc, int ktrim) {
+  if (skip && *skip) return;
+  gemm_rega_f64<false, true, true>(M, N, K, alpha, A, lda, B, ldb, beta, C, ldc, vecc, mode, sa, sb,
+                              sc, ktrim);
 }
 
 __global__ void __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(2, 2)))
@@ -547,7 +588,17 @@ gemm_tn_f64(int M, int N, int K, double alpha, const double* __restrict__ A, lon
             const double* B, long ldb, double beta, double* C, long ldc, int vecc, int mode,
             const int* __restrict__ skip) {
   if (skip && *skip) return;
-  gemm_rega_f64<true, false>(M, N, K, alpha, A, lda, B, ldb, beta, C, ldc, vecc, mode);
+  gemm_rega_f64<true, false, false>(M, N, K, alpha, A, lda, B, ldb, beta, C, ldc, vecc, mode, 0, 0,
+                               0, 0);
+}
+
+__global__ void __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(2, 2)))
+gemm_tn_f64_x(int M, int N, int K, double alpha, const double* __restrict__ A, long lda,
+            const double* B, long ldb, double beta, double* C, long ldc, int vecc, int mode,
+            const int* __restrict__ skip, long sa, long sb, long sc, int ktrim) {
+  if (skip && *skip) return;
+  gemm_rega_f64<true, false, true>(M, N, K, alpha, A, lda, B, ldb, beta, C, ldc, vecc, mode, sa, sb,
+                              sc, ktrim);
 }
 
 template <class S>
@@ -763,6 +814,30 @@ void gemm(hipStream_t s, int TA, int TB, long M, long N, long K, S alpha, const 
     launch_t<S, true, true>(s, g, va, vb, M, N, K, alpha, A, lda, B, ldb, beta, C, ldc, mode,
                             skip);
   ATLG_LAUNCHED();
+}
+
+bool gemm_batched(hipStream_t s, int TA, int TB, long M, long N, long K, double alpha,
+                  const double* A, long lda, long sa, const double* B, long ldb, long sb,
+                  double beta, double* C, long ldc, long sc, int batch, int mode, int ktrim,
+                  const int* skip) {
+  const bool at = is_trans(TA), bt = is_trans(TB);
+  if (M <= 0 || N <= 0 || K <= 0 || batch <= 0 || batch > 65535 || (at && bt) ||
+      K % (2 * BK) != 0 || ((sa | sb | sc) & 1) || !aligned16<double>(A, lda) ||
+      !aligned16<double>(B, ldb) || 8L * (at ? BM * lda + BK : 4 * lda + BM) >= (1L << 31) ||
+      8L * (bt ? BK * ldb + BN : BN * ldb + BK) >= (1L << 31))
+    return false;
+  // every tile's K range must be nonempty (the kernel's prologue loads its
+  // first K tiles unconditionally)
+  if (((ktrim & 1) && N > K) || ((ktrim & 2) && M > K) || ((ktrim & 4) && (ktrim & 3)))
+    return false;
+  const long tilesM = (M + BM - 1) / BM, tilesN = (N + BN - 1) / BN;
+  const long nwg = mode == 0 ? tilesM * tilesN : tri_tiles(mode, (int)tilesM, (int)tilesN);
+  const int vecc = aligned16<double>(C, ldc) ? 1 : 0;
+  hipLaunchKernelGGL(at ? gemm_tn_f64_x : (bt ? gemm_nt_f64_x : gemm_nn_f64_x),
+                     dim3((unsigned)nwg, (unsigned)batch), dim3(NT), 0, s, (int)M, (int)N, (int)K,
+                     alpha, A, lda, B, ldb, beta, C, ldc, vecc, mode, skip, sa, sb, sc, ktrim);
+  ATLG_LAUNCHED();
+  return true;
 }
 
 template void scale<double>(hipStream_t, long, long, double, double*, long, int, const int*);

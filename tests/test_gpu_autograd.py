@@ -121,3 +121,31 @@ def test_loglik_finite_differences(AG, n):
         noise = 2.0 * 1.1e-16 * n * abs(val.item()) / (h * max(abs(an), 1e-300))
         assert err <= 1e-4 + noise, (err, noise)
     print("N=%d: worst FD relative error over 20 directions %.2e (loglik %.6e)" % (n, worst, val.item()))
+
+
+def test_loglik_several_columns_matches_cpu_autograd(AG):
+    # y with k = 3 columns: the sum of the columns' log-likelihoods, the
+    # gradient in closed form against CPU FP64 autograd through
+    # torch.linalg.cholesky and a triangular solve
+    import torch
+    n, k = 700, 3
+    K, _ = rbf_problem(n, 6)
+    g = torch.Generator().manual_seed(6)
+    Y = torch.randn((n, k), generator=g, dtype=torch.float64).to(K.device)
+    Kc = K.detach().cpu().clone().requires_grad_(True)
+    Yc = Y.detach().cpu().clone().requires_grad_(True)
+    L = torch.linalg.cholesky(Kc)
+    a = torch.linalg.solve_triangular(L, Yc, upper=False)
+    ref = -0.5 * (a * a).sum() - k * torch.log(torch.diagonal(L)).sum() \
+        - 0.5 * n * k * math.log(2 * math.pi)
+    (2.0 * ref).backward()  # an upstream gradient other than 1
+    gK_ref = 0.5 * (Kc.grad + Kc.grad.t())
+    for store in (True, False):
+        Kg = K.clone().requires_grad_(True)
+        Yg = Y.clone().requires_grad_(True)
+        val = AG.gaussian_loglik(Kg, Yg, store_factor=store)
+        (2.0 * val).backward()
+        assert abs(val.item() - ref.item()) <= 1e-10 * abs(ref.item())
+        assert rel(Kg.grad.cpu(), gK_ref) <= 1e-9
+        assert rel(Yg.grad.cpu(), Yc.grad) <= 1e-9
+        assert torch.equal(Kg.grad, Kg.grad.t())

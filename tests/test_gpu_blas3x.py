@@ -201,13 +201,55 @@ def test_lauum(G, uplo, n):
     np.testing.assert_allclose(A[:n], ref, rtol=0, atol=1e-12 * np.abs(ref).max())
 
 
+@pytest.mark.parametrize("uplo", [O.Upper, O.Lower])
+@pytest.mark.parametrize("diag", [O.NonUnit, O.Unit])
+@pytest.mark.parametrize("n", [1024, 1500, 2300])
+def test_trtri_doubling_orders(G, uplo, diag, n):
+    # orders that take several doubling levels, with ragged last blocks at
+    # some of them (1500: 1500-1024 = 476 rows at level 2048; 2300), both
+    # triangles and diagonals: residual of inv(T) T, the other triangle
+    # and a unit diagonal untouched, agreement with the textbook inverse
+    rng = _rng(n + 7 * uplo + diag)
+    a = _well_tri(rng, n, uplo)
+    A = _ldpad(a)
+    assert G.clapack_dtrtri(G.ColMajor, uplo, diag, n, A, A.shape[0]) == 0
+    t = _tri(a, uplo, diag)
+    inv = _tri(A[:n], uplo, diag)
+    other = (np.tril(np.ones((n, n)), -1) if uplo == O.Upper else np.triu(np.ones((n, n)), 1))
+    assert np.array_equal(A[:n][other.astype(bool)], a[other.astype(bool)])
+    assert np.all(A[n:] == -1.0e10)
+    if diag == O.Unit:
+        assert np.array_equal(np.diag(A[:n]), np.diag(a))
+    resid = np.abs(inv @ t - np.eye(n)).max()
+    assert resid < 64 * n * 2.0 ** -52 * np.abs(inv).max() * np.abs(t).max() + 1e-15
+    ref = np.linalg.inv(t)
+    np.testing.assert_allclose(inv, ref, rtol=0, atol=1e-11 * np.abs(ref).max())
+
+
+@pytest.mark.parametrize("uplo", [O.Upper, O.Lower])
+@pytest.mark.parametrize("n", [1100, 2048])
+def test_lauum_masked_product_orders(G, uplo, n):
+    rng = _rng(90 + n + uplo)
+    a = rng.uniform(-1, 1, (n, n))
+    A = _ldpad(a)
+    G.clapack_dlauum(G.ColMajor, uplo, n, A, A.shape[0])
+    t = _tri(a, uplo, O.NonUnit)
+    full = t @ t.T if uplo == O.Upper else t.T @ t
+    tri = np.triu(np.ones((n, n), bool)) if uplo == O.Upper else np.tril(np.ones((n, n), bool))
+    assert np.array_equal(A[:n][~tri], a[~tri])
+    assert np.all(A[n:] == -1.0e10)
+    gauge = (np.abs(t) @ np.abs(t).T) if uplo == O.Upper else (np.abs(t).T @ np.abs(t))
+    ratio = np.abs(A[:n] - full)[tri] / (gauge[tri] * 2.0 ** -52)
+    assert ratio.max() < THRESH * max(1.0, np.sqrt(n))
+
+
 def _spd(rng, n):
     m = rng.uniform(-1, 1, (n, n))
     return m @ m.T / n + np.eye(n)
 
 
 @pytest.mark.parametrize("uplo", [O.Upper, O.Lower])
-@pytest.mark.parametrize("n", [5, 128, 900])
+@pytest.mark.parametrize("n", [5, 128, 900, 2100])
 def test_potri(G, uplo, n):
     rng = _rng(60 + n + uplo)
     s = _spd(rng, n)
