@@ -33,6 +33,7 @@
 
 #include "atlg_internal.hpp"
 #include "atlg_batched_tune.h"
+#include "tri_inv.hpp"
 
 namespace atlg {
 namespace {
@@ -67,12 +68,7 @@ struct Mfma<float> {
   static __device__ __forceinline__ int row(int l, int r) { return 4 * (l >> 4) + r; }
 };
 
-template <class S>
-__device__ __forceinline__ S ssqrt(S x);
-template <>
-__device__ __forceinline__ double ssqrt<double>(double x) { return sqrt(x); }
-template <>
-__device__ __forceinline__ float ssqrt<float>(float x) { return sqrtf(x); }
+static_assert(kBT == kTriThreads, "the in-LDS inverse takes the whole workgroup");
 
 __device__ __forceinline__ void wave_sync() {
   __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
@@ -97,48 +93,81 @@ struct alignas(16) Smem {
 };
 
 // ---- U: C = A[J:n, J:J+w] - L[J:n, 0:J] L[J:J+w, 0:J]^T (lower part) -----
+// The block row L[J:J+w, 0:J] that every row tile multiplies is staged in
+// LDS in chunks of kUC columns (the diagonal block's space, free between a
+// block column's TRSM and the next factor); each wave takes two 16-row
+// tiles per pass, so a staged chunk serves eight row tiles and the global
+// loads are the row tiles' own L rows.
+constexpr int kUC = 64;          // k per staged chunk
+constexpr int kUP = kTmax + 1;   // LDS pitch of a staged k row
+static_assert(kUC * kUP <= 2 * kTmax * kDP, "staging fits the diagonal block's space");
 template <class S, int T>
-__device__ void update_block(int n, S* __restrict__ A, int lda, int J, int w) {
+__device__ void update_block(int n, S* __restrict__ A, int lda, int J, int w, S* __restrict__ Ls) {
   typedef Mfma<S> M;
-  constexpr int CT = T / 16;
+  constexpr int CT = T / 16, NS = kUC * T / kBT;
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const int li = lane & 15, lk = lane >> 4;
   const int RT = (n - J + 15) >> 4;
-  const S* pc[CT];
-  bool cv[CT];
+  for (int p0 = 0; p0 < RT; p0 += 2 * kBW) {
+    int row[2];
+    bool rv[2];
+    const S* pr[2];
 #pragma unroll
-  for (int ct = 0; ct < CT; ct++) {
-    const int col = J + 16 * ct + li;
-    cv[ct] = col < J + w;
-    pc[ct] = A + (cv[ct] ? col : 0) + (size_t)lk * lda;
-  }
-  for (int rt = wave; rt < RT; rt += kBW) {
-    const int row0 = J + 16 * rt;
-    const int row = row0 + li;
-    const bool rv = row < n;
-    const S* pr = A + (rv ? row : 0) + (size_t)lk * lda;
-    typename M::acc_t acc[CT];
+    for (int h = 0; h < 2; h++) {
+      row[h] = J + 16 * (p0 + 2 * wave + h) + li;
+      rv[h] = row[h] < n;
+      pr[h] = A + (rv[h] ? row[h] : 0) + (size_t)lk * lda;
+    }
+    typename M::acc_t acc[2][CT];
 #pragma unroll
-    for (int ct = 0; ct < CT; ct++) acc[ct] = typename M::acc_t{0, 0, 0, 0};
-    // k = 4q + lk for q < J/4: every lane runs J/4 steps (J % 4 == 0)
-    const size_t step = (size_t)4 * lda;
-    size_t off = 0;
-#pragma unroll 4
-    for (int q = 0; q < (J >> 2); q++, off += step) {
-      const S x2 = rv ? pr[off] : S(0);
+    for (int h = 0; h < 2; h++)
 #pragma unroll
-      for (int ct = 0; ct < CT; ct++) {
-        const S x1 = cv[ct] ? pc[ct][off] : S(0);
-        acc[ct] = M::op(x1, x2, acc[ct]);
+      for (int ct = 0; ct < CT; ct++) acc[h][ct] = typename M::acc_t{0, 0, 0, 0};
+    for (int k0 = 0; k0 < J; k0 += kUC) {
+      const int kc = min(kUC, J - k0);  // a multiple of 4 (J is)
+      S v[NS];
+#pragma unroll
+      for (int u = 0; u < NS; u++) {
+        const int e = threadIdx.x + u * kBT, k = e / T, c = e % T;
+        v[u] = (k < kc && c < w) ? A[(J + c) + (size_t)(k0 + k) * lda] : S(0);
+      }
+      __syncthreads();  // the previous chunk's (or phase's) readers are done
+#pragma unroll
+      for (int u = 0; u < NS; u++) {
+        const int e = threadIdx.x + u * kBT;
+        Ls[(e / T) * kUP + e % T] = v[u];
+      }
+      __syncthreads();
+      // four k steps at a time (kc is a multiple of 16: J is)
+      for (int q0 = 0; q0 < (kc >> 2); q0 += 4) {
+        S x2[4][2];
+#pragma unroll
+        for (int dq = 0; dq < 4; dq++) {
+          const size_t off = (size_t)(k0 + 4 * (q0 + dq)) * lda;
+          x2[dq][0] = rv[0] ? pr[0][off] : S(0);
+          x2[dq][1] = rv[1] ? pr[1][off] : S(0);
+        }
+#pragma unroll
+        for (int dq = 0; dq < 4; dq++)
+#pragma unroll
+          for (int ct = 0; ct < CT; ct++) {
+            const S x1 = Ls[(4 * (q0 + dq) + lk) * kUP + 16 * ct + li];
+            acc[0][ct] = M::op(x1, x2[dq][0], acc[0][ct]);
+            acc[1][ct] = M::op(x1, x2[dq][1], acc[1][ct]);
+          }
       }
     }
 #pragma unroll
-    for (int ct = 0; ct < CT; ct++) {
-      if (row0 + 15 < J + 16 * ct) continue;  // tile wholly above the diagonal
+    for (int h = 0; h < 2; h++) {
+      const int row0 = J + 16 * (p0 + 2 * wave + h);
 #pragma unroll
-      for (int r = 0; r < 4; r++) {
-        const int col = J + 16 * ct + M::row(lane, r);
-        if (rv && col < J + w && row >= col) A[row + (size_t)col * lda] -= acc[ct][r];
+      for (int ct = 0; ct < CT; ct++) {
+        if (row0 + 15 < J + 16 * ct) continue;  // tile wholly above the diagonal
+#pragma unroll
+        for (int r = 0; r < 4; r++) {
+          const int col = J + 16 * ct + M::row(lane, r);
+          if (rv[h] && col < J + w && row[h] >= col) A[row[h] + (size_t)col * lda] -= acc[h][ct][r];
+        }
       }
     }
   }
@@ -168,44 +197,95 @@ __device__ __forceinline__ void load_diag(const S* __restrict__ A, int lda, int 
   }
 }
 
-// ---- F: factor the w x w diagonal block (wave 0) and invert it ------------
-// Returns 0, or j+1 for the first non-positive pivot.  D (pitch kDP) receives
-// L_JJ, V its inverse, zero padded to T x T; L_JJ is written back to A.
+// ---- F: factor the w x w diagonal block (wave 0) ---------------------------
+// Lane i holds row i of the block in registers and the factorization runs
+// column by column without barriers: the pivot from lane j by v_readlane,
+// the multipliers scaled by the reciprocal square root (hardware estimate
+// and Newton steps), column j broadcast to the lanes through a wave-private
+// LDS line (a wave's LDS accesses complete in order), read back eight at a
+// time.  Returns 0, or j+1 for the first non-positive pivot.  L_JJ goes back
+// to A, and to V (pitch kDP) padded with the identity to kTmax for the
+// in-LDS inverse that follows.
+template <class S>
+__device__ __forceinline__ S lane_val(S v, int src);
+template <>
+__device__ __forceinline__ double lane_val<double>(double v, int src) {
+  const long long u = __double_as_longlong(v);
+  const int lo = __builtin_amdgcn_readlane(static_cast<int>(u), src);
+  const int hi = __builtin_amdgcn_readlane(static_cast<int>(u >> 32), src);
+  return __longlong_as_double((static_cast<long long>(hi) << 32) | static_cast<unsigned>(lo));
+}
+template <>
+__device__ __forceinline__ float lane_val<float>(float v, int src) {
+  return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), src));
+}
+// 1/sqrt(d) for d > 0: the hardware estimate refined by Newton steps (two
+// from the FP64 estimate's ~2^-23, one in FP32); tiny d takes sqrt and a
+// division, where the estimate loses range.
+template <class S>
+__device__ __forceinline__ S rsqrt_p(S d);
+template <>
+__device__ __forceinline__ double rsqrt_p<double>(double d) {
+  if (d < 0x1p-1000) return 1.0 / sqrt(d);
+  double y = __builtin_amdgcn_rsq(d);
+  const double h = 0.5 * d;
+  y = y * (1.5 - h * y * y);
+  y = y * (1.5 - h * y * y);
+  return y;
+}
+template <>
+__device__ __forceinline__ float rsqrt_p<float>(float d) {
+  if (d < 0x1p-100f) return 1.0f / sqrtf(d);
+  float y = __builtin_amdgcn_rsqf(d);
+  y = y * (1.5f - 0.5f * d * y * y);
+  return y;
+}
+
 template <class S, int T>
-__device__ int factor_diag(int n, S* __restrict__ A, int lda, int J, int w, S* D, S* V) {
-  const int l = threadIdx.x;  // wave 0 only
-  load_diag<S, T>(A, lda, J, w, D);
-  for (int e = l; e < T * T; e += 64) {
-    const int i = e % T, c = e / T;
-    V[i * kDP + c] = (i == c) ? S(1) : S(0);
+__device__ int factor_diag(S* __restrict__ A, int lda, int J, int w, S* V, S* cb) {
+  constexpr int CB = 8;  // broadcast values read per batch
+  const int i = threadIdx.x;  // wave 0 only
+  S x[T];
+#pragma unroll
+  for (int c = 0; c < T; c++)
+    x[c] = (i < w && c <= i) ? A[(J + i) + (size_t)(J + c) * lda] : S(0);
+  int f = 0;  // uniform: the loop is fully unrolled, its branches uniform
+#pragma unroll
+  for (int j = 0; j < T; j++) {
+    if (j < w && f == 0) {
+      const S d = lane_val(x[j], j);
+      if (!(d > S(0))) {
+        f = j + 1;  // d is the same in every lane
+      } else {
+        const S y = rsqrt_p(d);
+        const S L = (i == j) ? d * y : x[j] * y;
+        if (i >= j) x[j] = L;
+        const S Lm = (i > j && i < w) ? L : S(0);  // lanes <= j: no update
+        cb[i] = L;
+#pragma unroll
+        for (int c0 = j + 1; c0 < T; c0 += CB) {
+          S cv[CB];
+#pragma unroll
+          for (int u = 0; u < CB; u++)
+            if (c0 + u < T) cv[u] = cb[c0 + u];
+#pragma unroll
+          for (int u = 0; u < CB; u++)
+            if (c0 + u < T) x[c0 + u] -= Lm * cv[u];
+        }
+      }
+    }
   }
-  wave_sync();
-  const int c = l & 31, h = l >> 5;
-  for (int j = 0; j < w; j++) {
-    const S d = D[j * kDP + j];
-    if (!(d > S(0))) return j + 1;  // d is the same in every lane
-    const S ljj = ssqrt(d), rinv = S(1) / ljj;
-    wave_sync();
-    if (l > j && l < w) D[l * kDP + j] *= rinv;
-    if (l == 0) D[j * kDP + j] = ljj;
-    wave_sync();
-    if (c > j)
-      for (int i = j + 1 + h; i < w; i += 2)
-        if (c <= i) D[i * kDP + c] -= D[i * kDP + j] * D[c * kDP + j];
-    wave_sync();
-  }
-  // Gauss-Jordan on [L | I]: V <- L^-1
-  for (int j = 0; j < w; j++) {
-    const S rinv = S(1) / D[j * kDP + j];
-    if (l <= j) V[j * kDP + l] *= rinv;
-    wave_sync();
-    if (c <= j)
-      for (int i = j + 1 + h; i < w; i += 2) V[i * kDP + c] -= D[i * kDP + j] * V[j * kDP + c];
-    wave_sync();
-  }
-  for (int e = l; e < w * w; e += 64) {
-    const int i = e % w, cc = e / w;
-    if (cc <= i) A[(J + i) + (size_t)(J + cc) * lda] = D[i * kDP + cc];
+  if (f) return f;
+#pragma unroll
+  for (int c = 0; c < T; c++)
+    if (i < w && c <= i) A[(J + i) + (size_t)(J + c) * lda] = x[c];
+  if (i < kTmax) {
+#pragma unroll
+    for (int c = 0; c < kTmax; c++) {
+      S v = (i == c) ? S(1) : S(0);
+      if (c < T && i < w && c <= i) v = x[c];
+      V[i * kDP + c] = v;
+    }
   }
   return 0;
 }
@@ -249,15 +329,16 @@ __device__ int chol_sys(int n, S* __restrict__ A, int lda, Smem& sm) {
   S* V = reinterpret_cast<S*>(sm.V);
   for (int J = 0; J < n; J += T) {
     const int w = min(T, n - J);
-    if (J > 0) update_block<S, T>(n, A, lda, J, w);
+    if (J > 0) update_block<S, T>(n, A, lda, J, w, D);  // stages in D and V's space
     __syncthreads();
     if (threadIdx.x < 64) {
-      const int f = factor_diag<S, T>(n, A, lda, J, w, D, V);
+      const int f = factor_diag<S, T>(A, lda, J, w, V, D);  // D: the broadcast line
       if (threadIdx.x == 0) sm.flag = f ? J + f : 0;
     }
     __syncthreads();
     const int f = sm.flag;
     if (f) return f;
+    lower_inverse_inplace<S, kTmax>(V, w);  // V <- inv(L_JJ), all threads
     trsm_block<S, T>(n, A, lda, J, w, V);
     __syncthreads();
   }
@@ -427,10 +508,12 @@ struct SysDesc {
 
 // mode: 0 factor, 1 factor + solve, 2 solve with a given factor.
 // Occupancy: the compiler's unconstrained allocation leaves two 256-thread workgroups per CU;
-// capping registers for 5 waves per SIMD costs a few spills and gains more than it loses
-// (10 000 uneven systems, n 64..1024: dposv 25.3 -> 21.2 ms; 4 waves 22.0, 6 waves 21.3).
+// capping registers trades spills for workgroups in flight, which hide the latency the phases
+// wait on.  10 000 uneven systems, n 16..512, 4 right-hand sides: 4 waves per SIMD 16.3 ms,
+// 3 waves 16.4, 5 waves 19.2 (before the register factor of the diagonal block, 5 waves was
+// the best: 21.2 ms; 25.3 unconstrained).
 template <class S>
-__global__ void __launch_bounds__(kBT) __attribute__((amdgpu_waves_per_eu(5)))
+__global__ void __launch_bounds__(kBT) __attribute__((amdgpu_waves_per_eu(4)))
 posv_batched_kernel(int nsys, const SysDesc<S>* __restrict__ desc, int nrhs, int mode,
                     int* __restrict__ info, unsigned* __restrict__ counter) {
   __shared__ Smem sm;
@@ -518,8 +601,8 @@ __device__ void residual(const MixDesc& d, int nrhs, Smem& sm, bool* conv, doubl
   }
 }
 
-// Same trade as above, measured separately (dsposv on the same 10 000 systems: 48.6 ms
-// unconstrained, 38.1 ms at 3 waves per SIMD, 31.7 ms at 4, 32.0 ms at 5).
+// Same trade as above, measured separately (dsposv on the same 10 000 systems: 26.7 ms at 4
+// waves per SIMD, 31.2 at 3, 29.5 at 5; 48.6 unconstrained before the staged update).
 __global__ void __launch_bounds__(kBT) __attribute__((amdgpu_waves_per_eu(4)))
 mixed_posv_kernel(int nsys, const MixDesc* __restrict__ desc, int nrhs, int itermax,
                   int* __restrict__ info, int* __restrict__ iter, unsigned* __restrict__ counter) {
