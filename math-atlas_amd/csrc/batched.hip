@@ -88,6 +88,7 @@ struct alignas(16) Smem {
   };
   double red[kBW * 4];      // reductions
   double bs[kTmax * 4];     // a solve's current block of solved values (first 4 right-hand sides)
+  double rd[kTmax];         // a solve's reciprocal pivots of the current diagonal block
   int flag;                 // failure index + 1, 0 = fine
   int cur;                  // system taken from the queue
 };
@@ -391,6 +392,7 @@ __device__ void solve_sys(int n, const S* __restrict__ L, int lda, S* __restrict
   if (n <= 0) return;
   S* D = reinterpret_cast<S*>(sm.D);
   S* BS = reinterpret_cast<S*>(sm.bs);  // solved block, [c][q] for q < 4
+  S* RD = reinterpret_cast<S*>(sm.rd);   // 1 / L_jj of the block
   const int t = threadIdx.x, l = t & 63;
   constexpr int RG = 64 / T;  // right-hand sides per wave pass
   const int i = l % T, g = l / T;
@@ -400,12 +402,14 @@ __device__ void solve_sys(int n, const S* __restrict__ L, int lda, S* __restrict
     if (t < 64) {
       load_diag<S, T>(L, lda, J, w, D);
       wave_sync();
+      if (l < w) RD[l] = S(1) / D[l * kDP + l];  // one division per pivot, not per step and lane
+      wave_sync();
       for (int r0 = 0; r0 < nrhs; r0 += RG) {
         const int r = r0 + g;
         const bool act = i < w && r < nrhs;
         S y = act ? B[(J + i) + (size_t)r * ldb] : S(0);
         for (int j = 0; j < w; j++) {
-          const S yj = __shfl(y, g * T + j, 64) / D[j * kDP + j];
+          const S yj = __shfl(y, g * T + j, 64) * RD[j];
           if (i == j) y = yj;
           else if (i > j) y -= D[i * kDP + j] * yj;
         }
@@ -440,12 +444,14 @@ __device__ void solve_sys(int n, const S* __restrict__ L, int lda, S* __restrict
     if (t < 64) {
       load_diag<S, T>(L, lda, J, w, D);
       wave_sync();
+      if (l < w) RD[l] = S(1) / D[l * kDP + l];
+      wave_sync();
       for (int r0 = 0; r0 < nrhs; r0 += RG) {
         const int r = r0 + g;
         const bool act = i < w && r < nrhs;
         S x = act ? B[(J + i) + (size_t)r * ldb] : S(0);
         for (int j = w - 1; j >= 0; j--) {
-          const S xj = __shfl(x, g * T + j, 64) / D[j * kDP + j];
+          const S xj = __shfl(x, g * T + j, 64) * RD[j];
           if (i == j) x = xj;
           else if (i < j) x -= D[j * kDP + i] * xj;
         }
