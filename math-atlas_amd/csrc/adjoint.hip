@@ -11,7 +11,8 @@
 // where Phi takes the lower triangle with the diagonal halved and
 // sym(X) = (X + X^T)/2 (Murray, "Differentiation of the Cholesky
 // decomposition", 2016, its blocked matrix form).  Every product runs on the
-// MFMA TRMM and TRSM of this library (3 n^3 flops for the potrf adjoint).
+// MFMA kernels of this library (5 n^3 / 3 flops for the potrf adjoint up to
+// the orders whose dense copies fit, 3 n^3 beyond).
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -40,6 +41,20 @@ __global__ void phi_kernel(long N, double* __restrict__ X, long ldx) {
     const long i = e % N, j = e / N;
     double* x = X + i + j * ldx;
     if (i < j) *x = 0.0;
+    else if (i == j) *x *= 0.5;
+  }
+}
+
+// X (order np, the first n rows and columns in use) <- Phi(X) with every
+// element outside the leading lower triangle zeroed, so that X can be a
+// triangular operand of the K-trimmed products
+__global__ void phi_padded_kernel(long n, long np, double* __restrict__ X) {
+  const long total = np * np;
+  for (long e = blockIdx.x * (long)blockDim.x + threadIdx.x; e < total;
+       e += (long)gridDim.x * blockDim.x) {
+    const long i = e % np, j = e / np;
+    double* x = X + e;
+    if (i < j || i >= n) *x = 0.0;
     else if (i == j) *x *= 0.5;
   }
 }
@@ -84,14 +99,51 @@ unsigned grid_for(long total) {
 
 }  // namespace
 
-// Kbar = sym(L^-T Phi(L^T Lbar) L^-1): P = L^T tril(Lbar) by TRMM, Phi(P),
-// then the two triangular solves with L's diagonal-block inverses computed
-// once, 3 n^3 flops in all (the GEMM form of L^T Lbar took 2 n^3 for that
-// step alone).  A solve that keeps Phi(P) L^-1 lower triangular saves half
-// of one solve's flops, but its recursion's many small launches made it
-// slower at n = 8192 (16 against 10.5 ms).  Only L's lower triangle is read.
+// Kbar = sym(L^-T Phi(L^T Lbar) L^-1).
+//
+// Up to the size where three dense n x n copies fit (the common case): L
+// and tril(Lbar) are copied into zero-padded dense matrices, then on the
+// register GEMM with each 128x128 tile's K range trimmed to where its
+// triangular operands can be nonzero:
+//   P = L^T tril(Lbar)   (lower part only: n^3/3 flops), P <- Phi(P);
+//   X = L^-1             (doubling inverse in place of the copy of L, n^3/3);
+//   Q = P X              (lower times lower, lower: n^3/3);
+//   R = X^T Q            (full: 2n^3/3), Kbar = sym(R).
+// 5n^3/3 flops against the 3n^3 of the TRMM + two-solve form below, which
+// serves larger orders.  Only L's lower triangle is read.
 void potrf_adjoint(hipStream_t s, long N, const double* L, long ldl, double* G, long ldg) {
   if (N <= 0) return;
+  const long np = tri_full_order(N);
+  const unsigned gp = grid_for(np * np);
+  if (N >= 2 * kTrsmNB && 3 * (size_t)np * np * sizeof(double) <= (size_t(6) << 30)) {
+    const size_t bytes = sizeof(double) * np * np;
+    double* X = static_cast<double*>(scratch(s, kSlotTriFull, bytes));
+    double* M = static_cast<double*>(scratch(s, kSlotAdjoint, bytes));
+    double* P = static_cast<double*>(scratch(s, kSlotAdjP, bytes));
+    tri_copy_to(s, true, false, N, L, ldl, X, nullptr);
+    tri_copy_to(s, true, false, N, G, ldg, M, nullptr);
+    // the register path applies: K = np is a multiple of 32, the copies are
+    // aligned with an even leading dimension, and every trimmed range is
+    // nonempty (N <= np)
+    if (!gemm_batched(s, kTrans, kNoTrans, N, N, np, 1.0, X, np, 0, M, np, 0, 0.0, P, np, 0, 1,
+                      1, 3, nullptr))
+      throw HipFailure{"potrf adjoint: the trimmed product does not apply"};
+    hipLaunchKernelGGL(phi_padded_kernel, dim3(gp), dim3(256), 0, s, N, np, P);
+    ATLG_LAUNCHED();
+    tri_invert_full(s, kNonUnit, N, X, np, nullptr);
+    if (!gemm_batched(s, kNoTrans, kNoTrans, N, N, np, 1.0, P, np, 0, X, np, 0, 0.0, M, np, 0, 1,
+                      1, 4 | 1, nullptr) ||
+        !gemm_batched(s, kTrans, kNoTrans, N, N, np, 1.0, X, np, 0, M, np, 0, 0.0, P, np, 0, 1,
+                      0, 3, nullptr))
+      throw HipFailure{"potrf adjoint: the trimmed product does not apply"};
+    const unsigned g = grid_for(N * N);
+    hipLaunchKernelGGL(sym_kernel, dim3(g), dim3(256), 0, s, N, P, np, G, ldg);
+    ATLG_LAUNCHED();
+    return;
+  }
+  // TRMM + two solves: P = L^T tril(Lbar) by TRMM, Phi(P), then the two
+  // triangular solves with L's diagonal-block inverses computed once, 3 n^3
+  // flops in all.
   constexpr long NB = kTrsmNB;
   const long nblk = (N + NB - 1) / NB;
   double* W = static_cast<double*>(

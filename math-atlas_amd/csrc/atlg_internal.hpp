@@ -101,7 +101,8 @@ enum Slot : int {
   kSlotSymm = 20,       // dense symmetric copy of A for SYMM
   kSlotTriFull = 21,    // dense zero-padded copy of a triangle (TRTRI, LAUUM, POTRI)
   kSlotTriTemp = 22,    // products of the doubling inverse
-  kNumSlots = 23
+  kSlotAdjP = 23,       // the potrf adjoint's product matrix
+  kNumSlots = 24
 };
 void* scratch(hipStream_t s, Slot slot, size_t bytes);
 
@@ -143,6 +144,15 @@ bool gemm_batched(hipStream_t s, int TA, int TB, long M, long N, long K, double 
                   const double* A, long lda, long sa, const double* B, long ldb, long sb,
                   double beta, double* C, long ldc, long sc, int batch, int mode, int ktrim,
                   const int* skip);
+// Dense zero-padded triangles (order rounded up to 32, leading dimension
+// that order): tri_copy_to writes op(A)'s triangle (transposed in when
+// upper; 1 on a unit diagonal) as a lower triangle with zeros elsewhere;
+// tri_invert_full inverts such a lower triangle in place (128-block
+// inverses, then doubling levels of batched, K-trimmed products).
+long tri_full_order(long n);
+void tri_copy_to(hipStream_t s, bool lower, bool unit, long n, const double* A, long lda,
+                 double* X, const int* skip);
+void tri_invert_full(hipStream_t s, int diag, long n, double* X, long np, const int* skip);
 // C <- beta*C on the M x N (or triangular per mode) part; beta==0 zeroes.
 template <class S>
 void scale(hipStream_t s, long M, long N, S beta, S* C, long ldc, int mode = 0,

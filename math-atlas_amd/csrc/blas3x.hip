@@ -252,59 +252,12 @@ tri_copy_out_kernel(int n, int np, const double* __restrict__ X, int lower, int 
   }
 }
 
-long tri_full_order(long n) { return (n + kCT - 1) / kCT * kCT; }
-bool tri_full_fits(long n) {
-  const long np = tri_full_order(n);
-  return n >= kTriFullMin && (size_t)np * np * sizeof(double) <= kTriFullCap;
-}
-
 double* tri_copy_in(hipStream_t s, bool lower, bool unit, long n, const double* A, long lda,
                     const int* skip) {
   const long np = tri_full_order(n);
   double* X = static_cast<double*>(scratch(s, kSlotTriFull, sizeof(double) * np * np));
-  const long tiles = np / kCT;
-  hipLaunchKernelGGL(tri_copy_in_kernel, dim3((unsigned)(tiles * tiles)), dim3(256), 0, s, (int)n,
-                     (int)np, A, lda, lower ? 1 : 0, unit ? 1 : 0, X, skip);
-  ATLG_LAUNCHED();
+  tri_copy_to(s, lower, unit, n, A, lda, X, skip);
   return X;
-}
-
-// X (order n, leading dimension np, lower, zeros elsewhere) <- inv(X):
-// the NB-blocks' inverses, then doubling: for blocks of order sz = 2NB,
-// 4NB, ... with halves already inverted, X21 <- -X22 (X21 X11).  The full
-// blocks of a level go in one batched launch per product, each product
-// restricted to the K range its triangular operand can reach; a ragged
-// last block takes two ordinary products (its zero padding keeps them exact).
-void tri_invert_full(hipStream_t s, int diag, long n, double* X, long np, const int* skip) {
-  const long nblk = (n + NB - 1) / NB;
-  double* D = static_cast<double*>(scratch(s, kSlotTrsmInv, sizeof(double) * nblk * NB * NB));
-  tri_inverse_blocks<double>(s, n, X, np, kNoTrans, kLower, diag, D, skip);
-  hipLaunchKernelGGL(put_tri_blocks_kernel, dim3((unsigned)nblk), dim3(256), 0, s, (int)n, X, np,
-                     1, 0, D, skip);
-  ATLG_LAUNCHED();
-  long tneed = 0;
-  for (long sz = 2 * NB; sz / 2 < n; sz *= 2) tneed = std::max(tneed, std::max(n / sz, 1L) * (sz / 2) * (sz / 2));
-  if (tneed == 0) return;
-  double* T = static_cast<double*>(scratch(s, kSlotTriTemp, sizeof(double) * tneed));
-  for (long sz = 2 * NB; sz / 2 < n; sz *= 2) {
-    const long h = sz / 2, nfull = n / sz, st = sz * (np + 1);
-    if (nfull > 0) {
-      // T_b = X21_b X11_b (X11 lower: k >= tile column), then
-      // X21_b = -X22_b T_b (X22 lower: k < tile row end)
-      if (!gemm_batched(s, kNoTrans, kNoTrans, h, h, h, 1.0, X + h, np, st, X, np, st, 0.0, T, h,
-                        h * h, (int)nfull, 0, 1, skip) ||
-          !gemm_batched(s, kNoTrans, kNoTrans, h, h, h, -1.0, X + h + h * np, np, st, T, h, h * h,
-                        0.0, X + h, np, st, (int)nfull, 0, 4, skip))
-        throw HipFailure{"trtri: the batched product does not apply"};
-    }
-    const long o = nfull * sz, r = n - o - h;
-    if (r > 0) {
-      gemm<double>(s, kNoTrans, kNoTrans, r, h, h, 1.0, X + o + h + o * np, np, X + o + o * np, np,
-                   0.0, T, h, 0, skip);
-      gemm<double>(s, kNoTrans, kNoTrans, r, h, r, -1.0, X + (o + h) + (o + h) * np, np, T, h, 0.0,
-                   X + o + h + o * np, np, 0, skip);
-    }
-  }
 }
 
 // A's triangle <- X^T X (lower: L^T L; upper, X = U^T: U U^T), one product
@@ -400,6 +353,60 @@ void trtri_rec(hipStream_t s, bool lower, int diag, long off, long N, double* A0
 }
 
 }  // namespace
+
+long tri_full_order(long n) { return (n + kCT - 1) / kCT * kCT; }
+
+static bool tri_full_fits(long n) {
+  const long np = tri_full_order(n);
+  return n >= kTriFullMin && (size_t)np * np * sizeof(double) <= kTriFullCap;
+}
+
+void tri_copy_to(hipStream_t s, bool lower, bool unit, long n, const double* A, long lda,
+                 double* X, const int* skip) {
+  const long np = tri_full_order(n);
+  const long tiles = np / kCT;
+  hipLaunchKernelGGL(tri_copy_in_kernel, dim3((unsigned)(tiles * tiles)), dim3(256), 0, s, (int)n,
+                     (int)np, A, lda, lower ? 1 : 0, unit ? 1 : 0, X, skip);
+  ATLG_LAUNCHED();
+}
+
+// X (order n, leading dimension np, lower, zeros elsewhere) <- inv(X):
+// the NB-blocks' inverses, then doubling: for blocks of order sz = 2NB,
+// 4NB, ... with halves already inverted, X21 <- -X22 (X21 X11).  The full
+// blocks of a level go in one batched launch per product, each product
+// restricted to the K range its triangular operand can reach; a ragged
+// last block takes two ordinary products (its zero padding keeps them exact).
+void tri_invert_full(hipStream_t s, int diag, long n, double* X, long np, const int* skip) {
+  const long nblk = (n + NB - 1) / NB;
+  double* D = static_cast<double*>(scratch(s, kSlotTrsmInv, sizeof(double) * nblk * NB * NB));
+  tri_inverse_blocks<double>(s, n, X, np, kNoTrans, kLower, diag, D, skip);
+  hipLaunchKernelGGL(put_tri_blocks_kernel, dim3((unsigned)nblk), dim3(256), 0, s, (int)n, X, np,
+                     1, 0, D, skip);
+  ATLG_LAUNCHED();
+  long tneed = 0;
+  for (long sz = 2 * NB; sz / 2 < n; sz *= 2) tneed = std::max(tneed, std::max(n / sz, 1L) * (sz / 2) * (sz / 2));
+  if (tneed == 0) return;
+  double* T = static_cast<double*>(scratch(s, kSlotTriTemp, sizeof(double) * tneed));
+  for (long sz = 2 * NB; sz / 2 < n; sz *= 2) {
+    const long h = sz / 2, nfull = n / sz, st = sz * (np + 1);
+    if (nfull > 0) {
+      // T_b = X21_b X11_b (X11 lower: k >= tile column), then
+      // X21_b = -X22_b T_b (X22 lower: k < tile row end)
+      if (!gemm_batched(s, kNoTrans, kNoTrans, h, h, h, 1.0, X + h, np, st, X, np, st, 0.0, T, h,
+                        h * h, (int)nfull, 0, 1, skip) ||
+          !gemm_batched(s, kNoTrans, kNoTrans, h, h, h, -1.0, X + h + h * np, np, st, T, h, h * h,
+                        0.0, X + h, np, st, (int)nfull, 0, 4, skip))
+        throw HipFailure{"trtri: the batched product does not apply"};
+    }
+    const long o = nfull * sz, r = n - o - h;
+    if (r > 0) {
+      gemm<double>(s, kNoTrans, kNoTrans, r, h, h, 1.0, X + o + h + o * np, np, X + o + o * np, np,
+                   0.0, T, h, 0, skip);
+      gemm<double>(s, kNoTrans, kNoTrans, r, h, r, -1.0, X + (o + h) + (o + h) * np, np, T, h, 0.0,
+                   X + o + h + o * np, np, 0, skip);
+    }
+  }
+}
 
 void trmm(hipStream_t s, int side, int uplo, int ta, int diag, long M, long N, double alpha,
           const double* A, long lda, double* B, long ldb, const int* skip) {

@@ -828,7 +828,10 @@ bool gemm_batched(hipStream_t s, int TA, int TB, long M, long N, long K, double 
     return false;
   // every tile's K range must be nonempty (the kernel's prologue loads its
   // first K tiles unconditionally)
-  if (((ktrim & 1) && N > K) || ((ktrim & 2) && M > K) || ((ktrim & 4) && (ktrim & 3)))
+  // (bit 4 with bit 1 only for the lower triangle of the output, whose
+  // tiles have n0 <= m0)
+  if (((ktrim & 1) && N > K) || ((ktrim & 2) && M > K) || ((ktrim & 4) && (ktrim & 2)) ||
+      ((ktrim & 4) && (ktrim & 1) && mode != 1))
     return false;
   const long tilesM = (M + BM - 1) / BM, tilesN = (N + BN - 1) / BN;
   const long nwg = mode == 0 ? tilesM * tilesN : tri_tiles(mode, (int)tilesM, (int)tilesN);

@@ -149,3 +149,23 @@ def test_loglik_several_columns_matches_cpu_autograd(AG):
         assert rel(Kg.grad.cpu(), gK_ref) <= 1e-9
         assert rel(Yg.grad.cpu(), Yc.grad) <= 1e-9
         assert torch.equal(Kg.grad, Kg.grad.t())
+
+
+@pytest.mark.parametrize("n", [300, 1500])
+def test_potrf_adjoint_matches_closed_form(AG, n):
+    # the primitives' chain (potrf adjoint with trimmed dense products, the
+    # few-RHS solve adjoint, logdet) against the fused node's closed-form
+    # gradient on the same problem, at orders with ragged 128-blocks
+    import torch
+    K, y = rbf_problem(n, 8)
+    Kg = K.clone().requires_grad_(True)
+    yg = y.clone().requires_grad_(True)
+    L = AG.cholesky(Kg)
+    a = AG.solve_lower(L, yg)
+    val = -0.5 * (a * a).sum() - 0.5 * AG.logdet_chol(L) - 0.5 * n * math.log(2 * math.pi)
+    val.backward()
+    Kf = K.clone().requires_grad_(True)
+    yf = y.clone().requires_grad_(True)
+    AG.gaussian_loglik(Kf, yf, store_factor=True).backward()
+    assert rel(Kg.grad, Kf.grad) <= 1e-9
+    assert rel(yg.grad, yf.grad) <= 1e-9
