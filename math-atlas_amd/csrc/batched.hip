@@ -385,38 +385,77 @@ __device__ __forceinline__ void row_dot1(const S* p, size_t st, int w, const S* 
   }
 }
 
+// Wave 0: solves the w x w diagonal block, L_JJ y = b (TR: L_JJ^T y = b),
+// for every right-hand side in place in B; the first four go to BS too.
+// Lane (i, g) works on row i and right-hand side r0 + g.  FP32: the block's
+// row (TR: column) i in registers, step j scales y_j by its reciprocal pivot
+// in lane j and shuffles it to the group.  FP64: the block in LDS with the
+// reciprocal pivots, which measured faster there (the registers of a row of
+// 32 doubles cost the kernel spills elsewhere).
+template <class S, int T, bool TR>
+__device__ void diag_solve(const S* __restrict__ L, int lda, int J, int w, S* __restrict__ B,
+                           int ldb, int nrhs, S* BS, Smem& sm) {
+  constexpr int RG = 64 / T;  // right-hand sides per pass
+  const int l = threadIdx.x & 63, i = l % T, g = l / T;
+  if constexpr (sizeof(S) == 4) {
+    S x[T];
+#pragma unroll
+    for (int c = 0; c < T; c++) {
+      const bool in = i < w && c < w && (TR ? c >= i : c <= i);
+      x[c] = in ? (TR ? L[(J + c) + (size_t)(J + i) * lda] : L[(J + i) + (size_t)(J + c) * lda])
+                : S(0);
+    }
+    const S rinv = i < w ? S(1) / L[(J + i) + (size_t)(J + i) * lda] : S(0);
+    for (int r0 = 0; r0 < nrhs; r0 += RG) {
+      const int r = r0 + g;
+      const bool act = i < w && r < nrhs;
+      S y = act ? B[(J + i) + (size_t)r * ldb] : S(0);
+#pragma unroll
+      for (int jj = 0; jj < T; jj++) {
+        const int j = TR ? T - 1 - jj : jj;
+        if (j < w) {
+          if (i == j) y *= rinv;
+          const S yj = __shfl(y, g * T + j, 64);
+          if (TR ? i < j : i > j) y -= x[j] * yj;
+        }
+      }
+      if (act) B[(J + i) + (size_t)r * ldb] = y;
+      if (act && r < 4) BS[i * 4 + r] = y;
+    }
+  } else {
+    S* D = reinterpret_cast<S*>(sm.D);
+    S* RD = reinterpret_cast<S*>(sm.rd);
+    load_diag<S, T>(L, lda, J, w, D);
+    wave_sync();
+    if (l < w) RD[l] = S(1) / D[l * kDP + l];  // one division per pivot, not per step and lane
+    wave_sync();
+    for (int r0 = 0; r0 < nrhs; r0 += RG) {
+      const int r = r0 + g;
+      const bool act = i < w && r < nrhs;
+      S y = act ? B[(J + i) + (size_t)r * ldb] : S(0);
+      for (int jj = 0; jj < w; jj++) {
+        const int j = TR ? w - 1 - jj : jj;
+        const S yj = __shfl(y, g * T + j, 64) * RD[j];
+        if (i == j) y = yj;
+        else if (TR ? i < j : i > j) y -= (TR ? D[j * kDP + i] : D[i * kDP + j]) * yj;
+      }
+      if (act) B[(J + i) + (size_t)r * ldb] = y;
+      if (act && r < 4) BS[i * 4 + r] = y;
+    }
+  }
+}
+
 // Solves L L^T X = B in place (B: n x nrhs, column-major, ldb).
 template <class S, int T>
 __device__ void solve_sys(int n, const S* __restrict__ L, int lda, S* __restrict__ B, int ldb,
                           int nrhs, Smem& sm) {
   if (n <= 0) return;
-  S* D = reinterpret_cast<S*>(sm.D);
   S* BS = reinterpret_cast<S*>(sm.bs);  // solved block, [c][q] for q < 4
-  S* RD = reinterpret_cast<S*>(sm.rd);   // 1 / L_jj of the block
-  const int t = threadIdx.x, l = t & 63;
-  constexpr int RG = 64 / T;  // right-hand sides per wave pass
-  const int i = l % T, g = l / T;
+  const int t = threadIdx.x;
   // forward: L Y = B
   for (int J = 0; J < n; J += T) {
     const int w = min(T, n - J);
-    if (t < 64) {
-      load_diag<S, T>(L, lda, J, w, D);
-      wave_sync();
-      if (l < w) RD[l] = S(1) / D[l * kDP + l];  // one division per pivot, not per step and lane
-      wave_sync();
-      for (int r0 = 0; r0 < nrhs; r0 += RG) {
-        const int r = r0 + g;
-        const bool act = i < w && r < nrhs;
-        S y = act ? B[(J + i) + (size_t)r * ldb] : S(0);
-        for (int j = 0; j < w; j++) {
-          const S yj = __shfl(y, g * T + j, 64) * RD[j];
-          if (i == j) y = yj;
-          else if (i > j) y -= D[i * kDP + j] * yj;
-        }
-        if (act) B[(J + i) + (size_t)r * ldb] = y;
-        if (act && r < 4) BS[i * 4 + r] = y;
-      }
-    }
+    if (t < 64) diag_solve<S, T, false>(L, lda, J, w, B, ldb, nrhs, BS, sm);
     __syncthreads();
     for (int row = J + w + t; row < n; row += kBT) {
       for (int r0 = 0; r0 < nrhs; r0 += 4) {
@@ -441,24 +480,7 @@ __device__ void solve_sys(int n, const S* __restrict__ L, int lda, S* __restrict
   // backward: L^T X = Y, last block row first
   for (int J = ((n - 1) / T) * T; J >= 0; J -= T) {
     const int w = min(T, n - J);
-    if (t < 64) {
-      load_diag<S, T>(L, lda, J, w, D);
-      wave_sync();
-      if (l < w) RD[l] = S(1) / D[l * kDP + l];
-      wave_sync();
-      for (int r0 = 0; r0 < nrhs; r0 += RG) {
-        const int r = r0 + g;
-        const bool act = i < w && r < nrhs;
-        S x = act ? B[(J + i) + (size_t)r * ldb] : S(0);
-        for (int j = w - 1; j >= 0; j--) {
-          const S xj = __shfl(x, g * T + j, 64) * RD[j];
-          if (i == j) x = xj;
-          else if (i < j) x -= D[j * kDP + i] * xj;
-        }
-        if (act) B[(J + i) + (size_t)r * ldb] = x;
-        if (act && r < 4) BS[i * 4 + r] = x;
-      }
-    }
+    if (t < 64) diag_solve<S, T, true>(L, lda, J, w, B, ldb, nrhs, BS, sm);
     __syncthreads();
     for (int k = t; k < J; k += kBT) {
       const S* lc = L + J + (size_t)k * lda;  // L[J:J+w, k], contiguous
