@@ -165,6 +165,33 @@ def sweep(torch, G, only="", ir_n=32768):
         out["dpotrf_16384_ms"] = ms
         out["dpotrf_16384_info"] = int(info.item())
         del s, s0
+    if want("inverses"):
+        # trtri / lauum / potri at 8192 on the lower Cholesky factor of
+        # B B^T / n + I (B iid normal), each from a fresh copy of the factor
+        n = 8192
+        bm = torch.empty((n, n), dtype=torch.float64, device="cuda")
+        g = torch.Generator(device="cuda")
+        g.manual_seed(51)
+        bm.normal_(generator=g)
+        f0 = torch.mm(bm, bm.t()).div_(n)
+        f0.diagonal().add_(1.0)
+        del bm
+        info = torch.zeros(1, dtype=torch.int32, device="cuda")
+        G.dpotrf(G.Lower, n, f0, n, info)
+        w = f0.clone()
+        ms_copy = timed(torch, lambda: w.copy_(f0), 2)
+        tri = n ** 3 / 3.0
+        for name, fn, fl in (
+                ("dtrtri", lambda: G.dtrtri(G.Lower, G.NonUnit, n, w, n, info), tri),
+                ("dlauum", lambda: G.dlauum(G.Lower, n, w, n), tri),
+                ("dpotri", lambda: G.dpotri(G.Lower, n, w, n, info), 2 * tri)):
+            def run():
+                w.copy_(f0)
+                fn()
+            ms = timed(torch, run, 2) - ms_copy
+            out["%s_8192_ms" % name] = ms
+            out["%s_8192_tflops" % name] = fl / ms / 1e9
+        del w, f0
     # the extensions report an error string instead of ending the run
     for key, fn in (("config1", lambda: bench_config1(G)),
                     ("ir", lambda: bench_ir(torch, G, ir_n)),

@@ -142,10 +142,10 @@ struct TileLoader {
 // one XCD (hardware deals ids round-robin over the 8 XCDs) cover a compact
 // block of tiles, sharing A row panels and B column panels in that XCD's L2.
 __device__ inline void tile_of(int bid, int nblk, int tilesM, int tilesN, int& tm,
-                               int& tn) {
+                               int& tn, bool compact = true) {
   const int per = nblk / 8, rem = nblk % 8;
   const int xcd = bid % 8, local = bid / 8;
-  const int id = !ATLG_GEMM_XCD ? bid
+  const int id = (!ATLG_GEMM_XCD || !compact) ? bid
                  : (xcd < rem) ? xcd * (per + 1) + local
                                : rem * (per + 1) + (xcd - rem) * per + local;
   constexpr int GM = ATLG_GEMM_GM;  // tile rows per group
@@ -343,13 +343,18 @@ __device__ __forceinline__ void gemm_rega_f64(int M, int N, int K, double alpha,
     B += (size_t)blockIdx.y * sb;
     C += (size_t)blockIdx.y * sc;
   }
+  // With K-trimmed tiles the work per tile follows its row or column, and
+  // the XCD-compact order (each XCD a contiguous range of tiles) would give
+  // one XCD all the long ones: those keep the dispatch order, which deals
+  // consecutive tiles to the XCDs in turn.
+  const bool compact = !(XT && ktrim != 0);
   for (int bid = blockIdx.x; bid < ntiles; bid += gridDim.x) {
     int tm, tn;
     if (mode == 0) {
-      tile_of(bid, ntiles, tilesM, tilesN, tm, tn);
+      tile_of(bid, ntiles, tilesM, tilesN, tm, tn, compact);
     } else {
       const int per = ntiles / 8, rem = ntiles % 8, xcd = bid % 8, local = bid / 8;
-      const int id = !ATLG_GEMM_XCD ? bid
+      const int id = (!ATLG_GEMM_XCD || !compact) ? bid
                      : (xcd < rem) ? xcd * (per + 1) + local
                                    : rem * (per + 1) + (xcd - rem) * per + local;
       tri_tile_of(id, mode, tilesM, tilesN, tm, tn);
