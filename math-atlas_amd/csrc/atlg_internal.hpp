@@ -61,7 +61,7 @@ hipStream_t library_stream();
 // priority) and kAuxEvents events, for work that overlaps the caller's stream
 // (getrf's look-ahead panel).  Created on first use.
 hipStream_t aux_stream();
-constexpr int kAuxEvents = 6;
+constexpr int kAuxEvents = 8;
 hipEvent_t aux_event(int i);  // i = 0 .. kAuxEvents-1
 // A pair of streams restricted to disjoint compute units: *small gets the
 // first `cus` units, *large the rest (hipExtStreamCreateWithCUMask).  Made

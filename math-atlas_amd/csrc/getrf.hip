@@ -40,6 +40,7 @@
 #include <cfloat>
 #include <climits>
 #include <limits>
+#include <type_traits>
 #include <vector>
 
 #include "atlg_internal.hpp"
@@ -124,6 +125,30 @@ constexpr long kLaMin = 2048;
 #ifndef ATLG_LA_SPLIT
 #define ATLG_LA_SPLIT 1
 #endif
+// Share of step k's trailing update that the panel units take on after the
+// next panel, while the update is the longer path (FP64; the low-precision
+// updates are fast enough that the panel always is).  Balance of the two
+// sides: the update side's rate ATLG_LA_UPD_TFLOPS on 192 units against a
+// third of it on the 64 panel units, and ATLG_LA_PANEL_US per panel column.
+#ifndef ATLG_LA_BALANCE
+#define ATLG_LA_BALANCE 1
+#endif
+#ifndef ATLG_LA_UPD_TFLOPS
+#define ATLG_LA_UPD_TFLOPS 55.0
+#endif
+#ifndef ATLG_LA_PANEL_US
+#define ATLG_LA_PANEL_US 7.0
+#endif
+template <class S>
+long la_chunk(long rest, long rows, long kb, long nb1, long nb2) {
+  if (!ATLG_LA_BALANCE || !std::is_same<S, double>::value || rows <= 0 || nb1 <= 0) return 0;
+  const double panel_s = ATLG_LA_PANEL_US * 1e-6 * (double)nb1;
+  const double per_col = 2.0 * (double)rows * (double)kb;  // flops per trailing column
+  const double c = ((double)rest - panel_s * ATLG_LA_UPD_TFLOPS * 1e12 / per_col) / 4.0;
+  long ci = c > 0 ? ((long)c / 128) * 128 : 0;
+  ci = std::min(ci, rest - nb2);  // never the next block's columns
+  return ci >= 256 ? ci : 0;
+}
 inline long la_split(long nb) {
   if (!ATLG_LA_SPLIT || nb < 2 * kTrsmNB) return 0;
   return ((nb / 2) / kTrsmNB) * kTrsmNB;
@@ -1753,8 +1778,9 @@ void getrf_lookahead(LU<S>& f0) {
   const long kLaNB = la_block(mn);
   f0.win_hi = std::min(kLaNB, mn);
   getrf_rec(f0, 0, f0.win_hi);
-  hipEvent_t e_left = aux_event(4);
+  hipEvent_t e_left = aux_event(4), e_trsm = aux_event(5), e_chunk = aux_event(6);
   long hk = 0;  // split column of block k within it (0: factored whole)
+  bool chunk_pending = false;  // the panel stream updates columns [N - chunk, N)
   for (long k0 = 0; k0 < mn; k0 += kLaNB) {
     const long kb = std::min(kLaNB, mn - k0), k1 = k0 + kb;
     const long nb1 = std::min(kLaNB, mn - k1);  // next block (none after the last)
@@ -1776,14 +1802,24 @@ void getrf_lookahead(LU<S>& f0) {
     // a split block's left half has not seen its right half's interchanges
     if (hk > 0) apply_block_lists(f0, k0 + hk, kb - hk, k0, hk, 0, 0);
     // The rest of step k is queued before the panel's many small launches,
-    // so s0 has work while the host issues them.
+    // so s0 has work while the host issues them.  While the update is the
+    // longer path, its last `chunk` columns go to the panel stream, after
+    // the next panel.
     const long r0 = k1 + nb1, rest = N - r0;
+    const long nb2n = std::min(kLaNB, std::max(0L, mn - r0));
+    const long chunk = (split && nb1 > 0 && M > k1) ? la_chunk<S>(rest, M - k1, kb, nb1, nb2n) : 0;
+    if (chunk_pending) {  // the previous step's share on the panel stream
+      ATLG_HIP(hipStreamWaitEvent(s0, e_chunk, 0));
+      chunk_pending = false;
+    }
     apply_block_lists(f0, k0, kb, r0, rest, 0, 0);
     if (rest > 0) {
       node_trsm_cols(f0, k0, kb, r0, rest);
+      if (chunk > 0) ATLG_HIP(hipEventRecord(e_trsm, s0));
       if (M > k1) {
         f0.gemm_cap = !split && (M - k1) * rest < ATLG_LA_CAP_AREA ? ATLG_LA_GEMM_CAP : 0;
-        node_update(f0, M - k1, rest, kb, at(f0, k1, k0), at(f0, k0, r0), at(f0, k1, r0));
+        node_update(f0, M - k1, rest - chunk, kb, at(f0, k1, k0), at(f0, k0, r0),
+                    at(f0, k1, r0));
         f0.gemm_cap = 0;
       }
     }
@@ -1812,6 +1848,13 @@ void getrf_lookahead(LU<S>& f0) {
         getrf_rec(f1, k1, nb1);
       }
       ATLG_HIP(hipEventRecord(e_panel, s1));
+      if (chunk > 0) {
+        const long c0 = N - chunk;
+        ATLG_HIP(hipStreamWaitEvent(s1, e_trsm, 0));
+        node_update(f1, M - k1, chunk, kb, at(f1, k1, k0), at(f1, k0, c0), at(f1, k1, c0));
+        ATLG_HIP(hipEventRecord(e_chunk, s1));
+        chunk_pending = true;
+      }
       if (h1 > 0 && nb2 > 0) {
         // the left half's share of the next block's update, beside the
         // right half's panel (s0 has finished the rest of step k)
@@ -1825,8 +1868,9 @@ void getrf_lookahead(LU<S>& f0) {
       hk = h1;
     }
   }
+  if (chunk_pending) ATLG_HIP(hipStreamWaitEvent(s0, e_chunk, 0));
   apply_deferred_left(f0, kLaNB, mn);
-  if (split) {  // every panel was joined into s0 above
+  if (split) {  // every panel and panel-side update was joined into s0 above
     ATLG_HIP(hipEventRecord(aux_event(3), s0));
     ATLG_HIP(hipStreamWaitEvent(caller, aux_event(3), 0));
     f0.s = caller;
