@@ -130,11 +130,15 @@ constexpr long kLaMin = 2048;
 // updates are fast enough that the panel always is).  Balance of the two
 // sides: the update side's rate ATLG_LA_UPD_TFLOPS on 192 units against a
 // third of it on the 64 panel units, and ATLG_LA_PANEL_US per panel column.
+// Measured (DGETRF, uniform [-1,1], ms at N=16384 / 32768): rate 40:
+// 122.1 / 497; 55: 123.6 / 504; 70: 124.4 / 513; 30: 123.7 / 517; 20: 130 /
+// 539; no balancing: 124 / 516 (the panel-time constant 5 instead of 7
+// with rate 55: 122.7 / 501).
 #ifndef ATLG_LA_BALANCE
 #define ATLG_LA_BALANCE 1
 #endif
 #ifndef ATLG_LA_UPD_TFLOPS
-#define ATLG_LA_UPD_TFLOPS 55.0
+#define ATLG_LA_UPD_TFLOPS 40.0
 #endif
 #ifndef ATLG_LA_PANEL_US
 #define ATLG_LA_PANEL_US 7.0
