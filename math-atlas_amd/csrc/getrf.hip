@@ -1743,7 +1743,10 @@ void apply_deferred_left(LU<S>& f, long NB, long mn) {
 // step 1 for the next block (interchanges, TRSM and a GEMM with K = half)
 // while the right half is factored, so that only the right half's share
 // waits for the whole panel; the left half receives the right half's
-// interchanges afterwards, before step 2 reads the block.
+// interchanges afterwards, before step 2 reads the block.  While step 2 is
+// the longer path, its GEMM leaves its last columns (la_chunk) to the panel
+// stream, which updates them after the panel; step k+1's step 2 waits for
+// them.
 // The columns each stream touches are disjoint while both run.  Partial
 // pivoting picks from the same updated columns as the plain recursion, so
 // the pivots agree wherever they are well separated.
