@@ -317,6 +317,26 @@ inline long chol_step(long N) {
   return N < 10240 ? 128 : (N < 14336 ? 256 : 384);
 }
 
+// Whether step k's panel solve (rest x nb1, after the diagonal block of nb1
+// columns) goes to the update side: when the rest of step k's trailing
+// update (rest^2 nb1 flops on the update units) takes less than the panel
+// side's leaves and solve (the solve on the ATLG_CHOL_PANEL_CUS panel units).
+#ifndef ATLG_CHOL_WIDE_SOLVE
+#define ATLG_CHOL_WIDE_SOLVE 1
+#endif
+#ifndef ATLG_CHOL_UPD_TFLOPS
+#define ATLG_CHOL_UPD_TFLOPS 50.0
+#endif
+inline bool chol_solve_wide(long rest, long nb1) {
+  if (!ATLG_CHOL_WIDE_SOLVE || rest <= 0) return false;
+  const double upd_rate = ATLG_CHOL_UPD_TFLOPS * 1e12,
+               panel_rate = upd_rate * ATLG_CHOL_PANEL_CUS / (256.0 - ATLG_CHOL_PANEL_CUS);
+  const double t_rest = (double)rest * (double)rest * (double)nb1 / upd_rate;
+  const double t_panel = (double)nb1 / 128.0 * 95e-6 +
+                         2.0 * (double)rest * (double)nb1 * (double)nb1 / panel_rate;
+  return t_rest < t_panel;
+}
+
 // Right-looking Cholesky by chol_step(N)-column blocks with one step of
 // look-ahead:
 // while the caller's stream applies step k's SYRK to the columns right of
@@ -380,9 +400,14 @@ void potrf_lookahead(hipStream_t s0, bool lower, long N, double* A, long lda, in
     update(s0, k0, kb, k1 + nb1, rest, rest, split ? 0 : ATLG_CHOL_CAP);
     ATLG_HIP(hipStreamWaitEvent(s1, e_upd, 0));
     potrf_rec(s1, lower, nb1, at(k1, k1), lda, dinfo, (int)k1, W);
-    solve_panel(s1, k1, nb1, rest);
+    // The panel solve is a GEMM-shaped TRSM of rest x nb1: on the panel
+    // units while the trailing update keeps the others busy, on the update
+    // side once that update is the shorter path (its units would wait).
+    const bool solve_wide = split && chol_solve_wide(rest, nb1);
+    if (!solve_wide) solve_panel(s1, k1, nb1, rest);
     ATLG_HIP(hipEventRecord(e_panel, s1));
     ATLG_HIP(hipStreamWaitEvent(s0, e_panel, 0));
+    if (solve_wide) solve_panel(s0, k1, nb1, rest);
   }
   if (split) {  // every panel was joined into s0 above
     ATLG_HIP(hipEventRecord(aux_event(3), s0));
