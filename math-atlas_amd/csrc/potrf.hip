@@ -306,9 +306,11 @@ constexpr long kCholLaMin = 2048;
 // Look-ahead step width (ATLG_CHOL_STEP when set, a multiple of 128): the
 // chain of diagonal blocks (three 128-block leaves and a panel solve per 384
 // columns) is the longer path at small orders, the trailing SYRK at large
-// ones.  Measured on MI355X (ms, panels on 32 compute units): N=8192 128:
-// 11.4, 256: 12.6, 512: 15.2; N=12288 256: 22.9; N=16384 256: 41.0, 384:
-// 39.8, 512: 43.7; 48 or 64 panel units were slower at every order.
+// ones.  Measured on MI355X (ms, panels on 32 compute units, panel solves
+// moved to the update units where chol_solve_wide says): N=8192 128: 10.9,
+// 256: 11.3, 512: 12.5; N=12288 128: 24.2, 256: 21.3, 512: 22.6; N=16384
+// 128: 47.7, 256: 39.4, 384: 37.3, 512: 38.1 (before the move, 384: 39.8);
+// 48 or 64 panel units were slower at every order.
 #ifndef ATLG_CHOL_STEP
 #define ATLG_CHOL_STEP 0
 #endif
