@@ -347,9 +347,8 @@ __device__ int chol_sys(int n, S* __restrict__ A, int lda, Smem& sm) {
 }
 
 // acc[q] += sum_{c < w} p[c*st] * b[c + (r0+q)*ldb] for the right-hand sides
-// r0 + q < nrhs, in order of c, four loads of p in flight at a time (used
-// by the mixed solver's kernels; the FP64 solver keeps one load at a time,
-// since the extra registers would cost its kernel an occupancy level).
+// r0 + q < nrhs, in order of c, four loads of p in flight at a time (the
+// mixed solver's FP64 residual).
 template <class S>
 __device__ __forceinline__ void row_dot4(const S* p, size_t st, int w, const S* b, int sc, int sq,
                                          int r0, int nrhs, S (&acc)[4]) {
@@ -366,18 +365,6 @@ __device__ __forceinline__ void row_dot4(const S* p, size_t st, int w, const S* 
         if (r0 + q < nrhs) acc[q] += lv[u] * b[(size_t)(c + u) * sc + (size_t)q * sq];
   }
   for (; c < w; c++) {
-    const S lv = p[(size_t)c * st];
-#pragma unroll
-    for (int q = 0; q < 4; q++)
-      if (r0 + q < nrhs) acc[q] += lv * b[(size_t)c * sc + (size_t)q * sq];
-  }
-}
-
-// The same one load at a time (the FP64 solver; see above).
-template <class S>
-__device__ __forceinline__ void row_dot1(const S* p, size_t st, int w, const S* b, int sc, int sq,
-                                         int r0, int nrhs, S (&acc)[4]) {
-  for (int c = 0; c < w; c++) {
     const S lv = p[(size_t)c * st];
 #pragma unroll
     for (int q = 0; q < 4; q++)
@@ -445,6 +432,49 @@ __device__ void diag_solve(const S* __restrict__ L, int lda, int J, int w, S* __
   }
 }
 
+// The rows outside the diagonal block take its solved values on the matrix
+// cores: forward, B[J+w:n] -= L[J+w:n, J:J+w] Y; backward (TR), B[0:J] -=
+// L[J:J+w, 0:J]^T X.  Each wave takes 16-row tiles and computes, four
+// right-hand sides at a time, the tile's update transposed (D = Y^T L^T), so
+// that a lane's result is one row of one right-hand side and the loads of L
+// cover 16 consecutive rows; all of a tile's loads are in flight together.
+template <class S, int T, bool TR>
+__device__ void offdiag_update(const S* __restrict__ L, int lda, int J, int w, int n,
+                               S* __restrict__ B, int ldb, int nrhs, const S* BS) {
+  typedef Mfma<S> M;
+  constexpr int KQ = T / 4;
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, li = lane & 15, lk = lane >> 4;
+  const int rbeg = TR ? 0 : J + w, rend = TR ? J : n;
+  const int RT = (rend - rbeg + 15) >> 4;
+  for (int r0 = 0; r0 < nrhs; r0 += 4) {
+    for (int rt = wave; rt < RT; rt += kBW) {
+      const int row = rbeg + 16 * rt + li;
+      const bool rv = row < rend;
+      S lv[KQ];
+#pragma unroll
+      for (int q = 0; q < KQ; q++) {
+        const int c = 4 * q + lk;
+        lv[q] = (rv && c < w) ? (TR ? L[(J + c) + (size_t)row * lda] : L[row + (size_t)(J + c) * lda])
+                              : S(0);
+      }
+      typename M::acc_t acc = {0, 0, 0, 0};
+#pragma unroll
+      for (int q = 0; q < KQ; q++) {
+        const int c = 4 * q + lk;
+        S yv = S(0);
+        if (li < 4 && c < w && r0 + li < nrhs)
+          yv = r0 == 0 ? BS[c * 4 + li] : B[(J + c) + (size_t)(r0 + li) * ldb];
+        acc = M::op(yv, lv[q], acc);
+      }
+#pragma unroll
+      for (int r = 0; r < 4; r++) {
+        const int q = M::row(lane, r);  // right-hand side within the group of four
+        if (rv && q < 4 && r0 + q < nrhs) B[row + (size_t)(r0 + q) * ldb] -= acc[r];
+      }
+    }
+  }
+}
+
 // Solves L L^T X = B in place (B: n x nrhs, column-major, ldb).
 template <class S, int T>
 __device__ void solve_sys(int n, const S* __restrict__ L, int lda, S* __restrict__ B, int ldb,
@@ -457,24 +487,7 @@ __device__ void solve_sys(int n, const S* __restrict__ L, int lda, S* __restrict
     const int w = min(T, n - J);
     if (t < 64) diag_solve<S, T, false>(L, lda, J, w, B, ldb, nrhs, BS, sm);
     __syncthreads();
-    for (int row = J + w + t; row < n; row += kBT) {
-      for (int r0 = 0; r0 < nrhs; r0 += 4) {
-        S acc[4] = {0, 0, 0, 0};
-        const S* lr = L + row + (size_t)J * lda;
-        // the block's solved values: from LDS for the first four right-hand sides
-        if (r0 == 0) {
-          if constexpr (sizeof(S) == 4) row_dot4<S>(lr, (size_t)lda, w, BS, 4, 1, r0, nrhs, acc);
-          else row_dot1<S>(lr, (size_t)lda, w, BS, 4, 1, r0, nrhs, acc);
-        } else {
-          const S* bj = B + J + (size_t)r0 * ldb;
-          if constexpr (sizeof(S) == 4) row_dot4<S>(lr, (size_t)lda, w, bj, 1, ldb, r0, nrhs, acc);
-          else row_dot1<S>(lr, (size_t)lda, w, bj, 1, ldb, r0, nrhs, acc);
-        }
-#pragma unroll
-        for (int q = 0; q < 4; q++)
-          if (r0 + q < nrhs) B[row + (size_t)(r0 + q) * ldb] -= acc[q];
-      }
-    }
+    offdiag_update<S, T, false>(L, lda, J, w, n, B, ldb, nrhs, BS);
     __syncthreads();
   }
   // backward: L^T X = Y, last block row first
@@ -482,23 +495,7 @@ __device__ void solve_sys(int n, const S* __restrict__ L, int lda, S* __restrict
     const int w = min(T, n - J);
     if (t < 64) diag_solve<S, T, true>(L, lda, J, w, B, ldb, nrhs, BS, sm);
     __syncthreads();
-    for (int k = t; k < J; k += kBT) {
-      const S* lc = L + J + (size_t)k * lda;  // L[J:J+w, k], contiguous
-      for (int r0 = 0; r0 < nrhs; r0 += 4) {
-        S acc[4] = {0, 0, 0, 0};
-        if (r0 == 0) {
-          if constexpr (sizeof(S) == 4) row_dot4<S>(lc, 1, w, BS, 4, 1, r0, nrhs, acc);
-          else row_dot1<S>(lc, 1, w, BS, 4, 1, r0, nrhs, acc);
-        } else {
-          const S* bj = B + J + (size_t)r0 * ldb;
-          if constexpr (sizeof(S) == 4) row_dot4<S>(lc, 1, w, bj, 1, ldb, r0, nrhs, acc);
-          else row_dot1<S>(lc, 1, w, bj, 1, ldb, r0, nrhs, acc);
-        }
-#pragma unroll
-        for (int q = 0; q < 4; q++)
-          if (r0 + q < nrhs) B[k + (size_t)(r0 + q) * ldb] -= acc[q];
-      }
-    }
+    offdiag_update<S, T, true>(L, lda, J, w, n, B, ldb, nrhs, BS);
     __syncthreads();
   }
 }
