@@ -133,6 +133,16 @@ template <class S>
 void gemm(hipStream_t s, int TA, int TB, long M, long N, long K, S alpha, const S* A, long lda,
           const S* B, long ldb, S beta, S* C, long ldc, int mode = 0,
           const int* skip = nullptr, int max_wg = 0);
+// C = alpha op(A) op(B) + beta C for K <= 128 and one short side: N <= 128
+// with A not transposed, or M <= 128 with B not transposed, by 32-wide
+// strips of the long side (4x the workgroups of the 128x128 tiling).  C must
+// not overlap A or B.  tri: 0 all of C, 1 only C(i,j) with i >= j, 2 only
+// i <= j.  `skip` as for gemm.  Returns false (launching nothing) when the
+// shape does not fit.
+template <class S>
+bool gemm_narrow(hipStream_t s, int TA, int TB, long M, long N, long K, S alpha, const S* A,
+                 long lda, const S* B, long ldb, S beta, S* C, long ldc, int tri = 0,
+                 const int* skip = nullptr);
 // FP64 register-path GEMM over `batch` problems whose operands sit at fixed
 // element strides (sa, sb, sc), optionally restricted per 128x128 tile to
 // the K range a triangular operand with a zero other triangle can reach

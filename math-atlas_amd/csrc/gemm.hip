@@ -632,21 +632,26 @@ __global__ void syreflect_kernel(int N, double* C, long ldc) {
   }
 }
 
-// In-place products with one short side, the diagonal-block applications
-// of TRSM and TRMM: C = alpha op(A) B + beta C with C == B and M, K <= 128
-// (LEFT), or C = alpha A op(B) + beta C with C == A and N, K <= 128.  The
-// 128x128 tiling would give them one workgroup per 128 columns (rows); here
-// a workgroup owns a 32-wide strip of the whole short side instead (4x the
-// workgroups), reads all of its strip (and op(A) / op(B), shared through L2)
-// before a barrier, and only then writes the strip back.  16x16 output tiles,
-// four per wave: LEFT, row tiles 2w, 2w+1 by both column tiles; else both
-// row tiles by column tiles 2w, 2w+1.  T: the short operand transposed.
+// Products with one short side: C = alpha op(A) B + beta C with M, K <= 128
+// (LEFT), or C = alpha A op(B) + beta C with N, K <= 128.  C may be B (LEFT)
+// or A (else): the diagonal-block applications of TRSM and TRMM, in place.
+// The 128x128 tiling would give them one workgroup per 128 columns (rows);
+// here a workgroup owns a 32-wide strip of the whole short side instead (4x
+// the workgroups), reads all of its strip (and op(A) / op(B), shared through
+// L2) before a barrier, and only then writes the strip back.  16x16 output
+// tiles, four per wave: LEFT, row tiles 2w, 2w+1 by both column tiles; else
+// both row tiles by column tiles 2w, 2w+1.  T: the short operand transposed.
+// tri: 0 all of C, 1 only C(i,j) with i >= j, 2 only i <= j (the stored
+// triangle of a symmetric diagonal block).  SL: 16-deep K slabs whose
+// operands are loaded per round trip; SL > 1 (for launches of fewer
+// workgroups than compute units, where each round trip's latency is the
+// cost) also loads the old C before the K loop.
 constexpr int kStrip = 32;
-template <class S, bool LEFT, bool T>
+template <class S, bool LEFT, bool T, int SL>
 __global__ void __launch_bounds__(256)
 gemm_strip(int M, int N, int K, S alpha, const S* __restrict__ A, long lda,
            const S* __restrict__ B, long ldb, S beta, S* C, long ldc,
-           const int* __restrict__ skip) {
+           const int* __restrict__ skip, int tri) {
   typedef Mfma<S> MF;
   if (skip && *skip) return;
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6, fx = lane & 15, fk = lane >> 4;
@@ -672,33 +677,41 @@ gemm_strip(int M, int N, int K, S alpha, const S* __restrict__ A, long lda,
   for (int i = 0; i < 2; i++)
 #pragma unroll
     for (int j = 0; j < 2; j++) acc[i][j] = typename MF::acc_t{0, 0, 0, 0};
-  for (int k0 = 0; k0 < K; k0 += 16) {  // four MFMA steps' operands in flight
-    S av[4][2], bv[4][2];
-#pragma unroll
-    for (int q = 0; q < 4; q++)
-#pragma unroll
-      for (int e = 0; e < 2; e++) {
-        av[q][e] = aval(rt[e] + fx, k0 + 4 * q + fk);
-        bv[q][e] = bval(k0 + 4 * q + fk, ct[e] + fx);
-      }
-#pragma unroll
-    for (int q = 0; q < 4; q++)
-#pragma unroll
-      for (int i = 0; i < 2; i++)
-#pragma unroll
-        for (int j = 0; j < 2; j++) acc[i][j] = MF::op(bv[q][j], av[q][i], acc[i][j]);
-  }
   // lane l, register r: C[rt + (l & 15)][ct + c_n(l, r)]
   S old[2][2][4];
+  auto load_old = [&]() {
 #pragma unroll
-  for (int i = 0; i < 2; i++)
+    for (int i = 0; i < 2; i++)
 #pragma unroll
-    for (int j = 0; j < 2; j++)
+      for (int j = 0; j < 2; j++)
 #pragma unroll
-      for (int r = 0; r < 4; r++) {
-        const int gi = rt[i] + fx, gj = ct[j] + MF::c_n(lane, r);
-        old[i][j][r] = (beta != S(0) && gi < M && gj < N) ? C[gi + (size_t)gj * ldc] : S(0);
-      }
+        for (int r = 0; r < 4; r++) {
+          const int gi = rt[i] + fx, gj = ct[j] + MF::c_n(lane, r);
+          old[i][j][r] = (beta != S(0) && gi < M && gj < N) ? C[gi + (size_t)gj * ldc] : S(0);
+        }
+  };
+  if (SL > 1) load_old();
+  for (int k0 = 0; k0 < K; k0 += 16 * SL) {  // 4 SL MFMA steps' operands in flight
+    S av[SL][4][2], bv[SL][4][2];
+#pragma unroll
+    for (int l = 0; l < SL; l++)
+#pragma unroll
+      for (int q = 0; q < 4; q++)
+#pragma unroll
+        for (int e = 0; e < 2; e++) {
+          av[l][q][e] = aval(rt[e] + fx, k0 + 16 * l + 4 * q + fk);
+          bv[l][q][e] = bval(k0 + 16 * l + 4 * q + fk, ct[e] + fx);
+        }
+#pragma unroll
+    for (int l = 0; l < SL; l++)
+#pragma unroll
+      for (int q = 0; q < 4; q++)
+#pragma unroll
+        for (int i = 0; i < 2; i++)
+#pragma unroll
+          for (int j = 0; j < 2; j++) acc[i][j] = MF::op(bv[l][q][j], av[l][q][i], acc[i][j]);
+  }
+  if (SL == 1) load_old();
   __syncthreads();  // every read of the strip precedes every write
 #pragma unroll
   for (int i = 0; i < 2; i++)
@@ -707,10 +720,42 @@ gemm_strip(int M, int N, int K, S alpha, const S* __restrict__ A, long lda,
 #pragma unroll
       for (int r = 0; r < 4; r++) {
         const int gi = rt[i] + fx, gj = ct[j] + MF::c_n(lane, r);
-        if (gi < M && gj < N)
+        if (gi < M && gj < N && !(tri == 1 && gi < gj) && !(tri == 2 && gi > gj))
           C[gi + (size_t)gj * ldc] =
               (beta == S(0)) ? alpha * acc[i][j][r] : alpha * acc[i][j][r] + beta * old[i][j][r];
       }
+}
+
+// Strip launches of fewer workgroups than this (a fraction of the device's
+// 256 compute units, latency-bound) load four K slabs per round trip.
+#ifndef ATLG_STRIP_DEEP_MAX
+#define ATLG_STRIP_DEEP_MAX 256
+#endif
+template <class S>
+void launch_strip(hipStream_t s, bool left, bool t, long M, long N, long K, S alpha, const S* A,
+                  long lda, const S* B, long ldb, S beta, S* C, long ldc, const int* skip,
+                  int tri) {
+  const long nstrips = ((left ? N : M) + kStrip - 1) / kStrip;
+  const dim3 gs((unsigned)nstrips);
+  const bool deep = nstrips < ATLG_STRIP_DEEP_MAX;
+#define ATLG_STRIP(L_, T_)                                                                   \
+  do {                                                                                       \
+    if (deep)                                                                                \
+      hipLaunchKernelGGL((gemm_strip<S, L_, T_, 4>), gs, dim3(256), 0, s, (int)M, (int)N,   \
+                         (int)K, alpha, A, lda, B, ldb, beta, C, ldc, skip, tri);            \
+    else                                                                                     \
+      hipLaunchKernelGGL((gemm_strip<S, L_, T_, 1>), gs, dim3(256), 0, s, (int)M, (int)N,   \
+                         (int)K, alpha, A, lda, B, ldb, beta, C, ldc, skip, tri);            \
+  } while (0)
+  if (left) {
+    if (t) ATLG_STRIP(true, true);
+    else ATLG_STRIP(true, false);
+  } else {
+    if (t) ATLG_STRIP(false, true);
+    else ATLG_STRIP(false, false);
+  }
+#undef ATLG_STRIP
+  ATLG_LAUNCHED();
 }
 
 template <class S, bool AT, bool BT>
@@ -766,25 +811,11 @@ void gemm(hipStream_t s, int TA, int TB, long M, long N, long K, S alpha, const 
   // in-place diagonal-block applications (C is B, or C is A) with a short
   // side of at most 128: strips of 32 across the long side
   if (mode == 0 && C == B && C != A && M <= BM && K <= BM && !bt) {
-    const dim3 gs((unsigned)((N + kStrip - 1) / kStrip));
-    if (at)
-      hipLaunchKernelGGL((gemm_strip<S, true, true>), gs, dim3(256), 0, s, (int)M, (int)N, (int)K,
-                         alpha, A, lda, B, ldb, beta, C, ldc, skip);
-    else
-      hipLaunchKernelGGL((gemm_strip<S, true, false>), gs, dim3(256), 0, s, (int)M, (int)N,
-                         (int)K, alpha, A, lda, B, ldb, beta, C, ldc, skip);
-    ATLG_LAUNCHED();
+    launch_strip<S>(s, true, at, M, N, K, alpha, A, lda, B, ldb, beta, C, ldc, skip, 0);
     return;
   }
   if (mode == 0 && C == A && C != B && N <= BN && K <= BN && !at) {
-    const dim3 gs((unsigned)((M + kStrip - 1) / kStrip));
-    if (bt)
-      hipLaunchKernelGGL((gemm_strip<S, false, true>), gs, dim3(256), 0, s, (int)M, (int)N,
-                         (int)K, alpha, A, lda, B, ldb, beta, C, ldc, skip);
-    else
-      hipLaunchKernelGGL((gemm_strip<S, false, false>), gs, dim3(256), 0, s, (int)M, (int)N,
-                         (int)K, alpha, A, lda, B, ldb, beta, C, ldc, skip);
-    ATLG_LAUNCHED();
+    launch_strip<S>(s, false, bt, M, N, K, alpha, A, lda, B, ldb, beta, C, ldc, skip, 0);
     return;
   }
   const bool va = aligned16<S>(A, lda), vb = aligned16<S>(B, ldb);
@@ -821,6 +852,21 @@ void gemm(hipStream_t s, int TA, int TB, long M, long N, long K, S alpha, const 
   ATLG_LAUNCHED();
 }
 
+template <class S>
+bool gemm_narrow(hipStream_t s, int TA, int TB, long M, long N, long K, S alpha, const S* A,
+                 long lda, const S* B, long ldb, S beta, S* C, long ldc, int tri,
+                 const int* skip) {
+  const bool at = is_trans(TA), bt = is_trans(TB);
+  if (M <= 0 || N <= 0 || K <= 0 || K > BM || M > (1L << 30) || N > (1L << 30)) return false;
+  if (N <= BN && !at)  // strips of 32 rows by all N columns
+    launch_strip<S>(s, false, bt, M, N, K, alpha, A, lda, B, ldb, beta, C, ldc, skip, tri);
+  else if (M <= BM && !bt)  // strips of 32 columns by all M rows
+    launch_strip<S>(s, true, at, M, N, K, alpha, A, lda, B, ldb, beta, C, ldc, skip, tri);
+  else
+    return false;
+  return true;
+}
+
 bool gemm_batched(hipStream_t s, int TA, int TB, long M, long N, long K, double alpha,
                   const double* A, long lda, long sa, const double* B, long ldb, long sb,
                   double beta, double* C, long ldc, long sc, int batch, int mode, int ktrim,
@@ -854,5 +900,10 @@ template void gemm<double>(hipStream_t, int, int, long, long, long, double, cons
                            const double*, long, double, double*, long, int, const int*, int);
 template void gemm<float>(hipStream_t, int, int, long, long, long, float, const float*, long,
                           const float*, long, float, float*, long, int, const int*, int);
+template bool gemm_narrow<double>(hipStream_t, int, int, long, long, long, double, const double*,
+                                  long, const double*, long, double, double*, long, int,
+                                  const int*);
+template bool gemm_narrow<float>(hipStream_t, int, int, long, long, long, float, const float*,
+                                 long, const float*, long, float, float*, long, int, const int*);
 
 }  // namespace atlg

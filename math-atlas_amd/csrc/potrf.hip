@@ -314,29 +314,51 @@ constexpr long kCholLaMin = 2048;
 #ifndef ATLG_CHOL_STEP
 #define ATLG_CHOL_STEP 0
 #endif
+// Widest block (columns) whose look-ahead update goes in two parts, the
+// diagonal block's triangle and then the rest of the block column, both by
+// 32-row strips (gemm_narrow): the leaf starts after the first part.  N=8192
+// (128-column steps): 10.9 -> 10.3 ms; for the 256/384-column steps of
+// N=12288/16384 the split, by the 128x128 tiling, was 1 ms slower.
+#ifndef ATLG_CHOL_SPLIT_MAX
+#define ATLG_CHOL_SPLIT_MAX 128
+#endif
 inline long chol_step(long N) {
   if (ATLG_CHOL_STEP > 0) return ATLG_CHOL_STEP;
   return N < 10240 ? 128 : (N < 14336 ? 256 : 384);
 }
 
 // Whether step k's panel solve (rest x nb1, after the diagonal block of nb1
-// columns) goes to the update side: when the rest of step k's trailing
-// update (rest^2 nb1 flops on the update units) takes less than the panel
-// side's leaves and solve (the solve on the ATLG_CHOL_PANEL_CUS panel units).
+// columns) goes to the update side.  Per step, the update side runs the
+// block column's update and the trailing SYRK (U), the panel side the leaves
+// (P) and, when narrow, the solve on its ATLG_CHOL_PANEL_CUS units (Sn); a
+// wide solve (Sw) runs after U on the update side instead.  The step takes
+// max(U, P + Sn) narrow and max(U, P) + Sw wide, so the solve goes wide when
+// U < P + Sn - Sw: the test for blocks of at most ATLG_CHOL_MODEL2_MAX
+// columns, with 32 TFLOP/s for their 128-deep updates (measured at
+// N = 8192; 26 and 40 were slower).  Wider blocks keep the simpler test
+// T < P + Sn (T the trailing SYRK alone) with the 50 TFLOP/s found by a
+// sweep at N = 16384 (the full test measured the same there).
 #ifndef ATLG_CHOL_WIDE_SOLVE
 #define ATLG_CHOL_WIDE_SOLVE 1
 #endif
 #ifndef ATLG_CHOL_UPD_TFLOPS
 #define ATLG_CHOL_UPD_TFLOPS 50.0
 #endif
+#ifndef ATLG_CHOL_UPD128_TFLOPS
+#define ATLG_CHOL_UPD128_TFLOPS 32.0
+#endif
+#ifndef ATLG_CHOL_MODEL2_MAX
+#define ATLG_CHOL_MODEL2_MAX 128
+#endif
 inline bool chol_solve_wide(long rest, long nb1) {
   if (!ATLG_CHOL_WIDE_SOLVE || rest <= 0) return false;
-  const double upd_rate = ATLG_CHOL_UPD_TFLOPS * 1e12,
+  const double upd_rate = (nb1 <= 128 ? ATLG_CHOL_UPD128_TFLOPS : ATLG_CHOL_UPD_TFLOPS) * 1e12,
                panel_rate = upd_rate * ATLG_CHOL_PANEL_CUS / (256.0 - ATLG_CHOL_PANEL_CUS);
-  const double t_rest = (double)rest * (double)rest * (double)nb1 / upd_rate;
-  const double t_panel = (double)nb1 / 128.0 * 95e-6 +
-                         2.0 * (double)rest * (double)nb1 * (double)nb1 / panel_rate;
-  return t_rest < t_panel;
+  const double r = (double)rest, b = (double)nb1;
+  const double t_solve_n = 2.0 * r * b * b / panel_rate, t_solve_w = 2.0 * r * b * b / upd_rate;
+  if (nb1 > ATLG_CHOL_MODEL2_MAX) return r * r * b / upd_rate < b / 128.0 * 95e-6 + t_solve_n;
+  const double t_upd = r * r * b / upd_rate + 2.0 * r * b * b / upd_rate;
+  return t_upd < b / 128.0 * 95e-6 + t_solve_n - t_solve_w;
 }
 
 // Right-looking Cholesky by chol_step(N)-column blocks with one step of
@@ -388,6 +410,33 @@ void potrf_lookahead(hipStream_t s0, bool lower, long N, double* A, long lda, in
            cap);
     }
   };
+  // the off-diagonal part of block column c0 (nc wide, r rows/columns past
+  // it) minus the product of block k's panel rows: narrow blocks by strips
+  auto update_col = [&](hipStream_t s, long k, long kb, long c0, long nc, long r) {
+    if (r <= 0) return;
+    if (lower) {
+      if (!gemm_narrow(s, kNoTrans, kTrans, r, nc, kb, -1.0, at(c0 + nc, k), lda, at(c0, k), lda,
+                       1.0, at(c0 + nc, c0), lda, 0, dinfo))
+        gemm(s, kNoTrans, kTrans, r, nc, kb, -1.0, at(c0 + nc, k), lda, at(c0, k), lda, 1.0,
+             at(c0 + nc, c0), lda, 0, dinfo);
+    } else {
+      if (!gemm_narrow(s, kTrans, kNoTrans, nc, r, kb, -1.0, at(k, c0), lda, at(k, c0 + nc), lda,
+                       1.0, at(c0, c0 + nc), lda, 0, dinfo))
+        gemm(s, kTrans, kNoTrans, nc, r, kb, -1.0, at(k, c0), lda, at(k, c0 + nc), lda, 1.0,
+             at(c0, c0 + nc), lda, 0, dinfo);
+    }
+  };
+  // the stored triangle of diagonal block c0 (nc wide) likewise
+  auto update_diag = [&](hipStream_t s, long k, long kb, long c0, long nc) {
+    const double* P = lower ? at(c0, k) : at(k, c0);
+    if (lower ? gemm_narrow(s, kNoTrans, kTrans, nc, nc, kb, -1.0, P, lda, P, lda, 1.0,
+                            at(c0, c0), lda, 1, dinfo)
+              : gemm_narrow(s, kTrans, kNoTrans, nc, nc, kb, -1.0, P, lda, P, lda, 1.0,
+                            at(c0, c0), lda, 2, dinfo))
+      return;
+    update(s, k, kb, c0, nc, nc, 0);
+  };
+  hipEvent_t e_col = aux_event(4);
   const long rb = chol_step(N);
   const long kb0 = std::min(rb, N);
   potrf_rec(s0, lower, kb0, A, lda, dinfo, 0, W);
@@ -395,10 +444,20 @@ void potrf_lookahead(hipStream_t s0, bool lower, long N, double* A, long lda, in
   for (long k0 = 0; k0 + kb0 < N; k0 += rb) {
     const long kb = std::min(rb, N - k0), k1 = k0 + kb, nb1 = std::min(rb, N - k1);
     const long rest = N - k1 - nb1;
-    // block k+1 first (all its rows), then the rest of step k's update,
-    // beside which the auxiliary stream factors block k+1 and its panel
-    update(s0, k0, kb, k1, N - k1, nb1, 0);
-    ATLG_HIP(hipEventRecord(e_upd, s0));
+    // block k+1 first, then the rest of step k's update, beside which the
+    // auxiliary stream factors block k+1 and solves its panel; for narrow
+    // blocks its diagonal block alone goes first (all the leaf needs), then
+    // the rest of its block column (what the panel solve needs)
+    const bool diag_first = nb1 <= ATLG_CHOL_SPLIT_MAX;
+    if (diag_first) {
+      update_diag(s0, k0, kb, k1, nb1);
+      ATLG_HIP(hipEventRecord(e_upd, s0));
+      update_col(s0, k0, kb, k1, nb1, rest);
+      ATLG_HIP(hipEventRecord(e_col, s0));
+    } else {
+      update(s0, k0, kb, k1, N - k1, nb1, 0);
+      ATLG_HIP(hipEventRecord(e_upd, s0));
+    }
     update(s0, k0, kb, k1 + nb1, rest, rest, split ? 0 : ATLG_CHOL_CAP);
     ATLG_HIP(hipStreamWaitEvent(s1, e_upd, 0));
     potrf_rec(s1, lower, nb1, at(k1, k1), lda, dinfo, (int)k1, W);
@@ -406,7 +465,10 @@ void potrf_lookahead(hipStream_t s0, bool lower, long N, double* A, long lda, in
     // units while the trailing update keeps the others busy, on the update
     // side once that update is the shorter path (its units would wait).
     const bool solve_wide = split && chol_solve_wide(rest, nb1);
-    if (!solve_wide) solve_panel(s1, k1, nb1, rest);
+    if (!solve_wide) {
+      if (diag_first) ATLG_HIP(hipStreamWaitEvent(s1, e_col, 0));
+      solve_panel(s1, k1, nb1, rest);
+    }
     ATLG_HIP(hipEventRecord(e_panel, s1));
     ATLG_HIP(hipStreamWaitEvent(s0, e_panel, 0));
     if (solve_wide) solve_panel(s0, k1, nb1, rest);
