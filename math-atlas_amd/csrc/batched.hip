@@ -77,7 +77,10 @@ __device__ __forceinline__ void wave_sync() {
 }
 
 // LDS of one workgroup.  `D` and `V` are typed per scalar at use; the
-// mixed solver's residual uses the same space as `XS`.
+// mixed solver's residual uses the same space as `XS`.  `Y` holds up to four
+// right-hand sides of a system of order <= kYmax / 4 while its factorization
+// runs the forward substitution alongside.
+constexpr int kYmax = 2048;
 struct alignas(16) Smem {
   union {
     struct {
@@ -87,6 +90,7 @@ struct alignas(16) Smem {
     double XS[2 * kTmax * kDP];  // residual: X[0:n, r0:r0+4] as [row][q]
   };
   double red[kBW * 4];      // reductions
+  double Y[kYmax];          // a fused forward solve's right-hand sides, [row][q] for q < 4
   double bs[kTmax * 4];     // a solve's current block of solved values (first 4 right-hand sides)
   double rd[kTmax];         // a solve's reciprocal pivots of the current diagonal block
   int flag;                 // failure index + 1, 0 = fine
@@ -292,8 +296,13 @@ __device__ int factor_diag(S* __restrict__ A, int lda, int J, int w, S* V, S* cb
 }
 
 // ---- T: A[J+w:n, J:J+w] <- A[J+w:n, J:J+w] * V^T ---------------------------
+// With Y (a fused forward solve, y_J already in Y[J:J+w]): also
+// Y[J+w:n] -= A[J+w:n, J:J+w] y_J, from the new rows still in registers; the
+// four lanes of a row each hold a quarter of its columns, summed by two
+// shuffles, and lane quarter q writes right-hand side q.
 template <class S, int T>
-__device__ void trsm_block(int n, S* __restrict__ A, int lda, int J, int w, const S* V) {
+__device__ void trsm_block(int n, S* __restrict__ A, int lda, int J, int w, const S* V,
+                           S* __restrict__ Y) {
   typedef Mfma<S> M;
   constexpr int CT = T / 16, KQ = T / 4;
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
@@ -309,6 +318,7 @@ __device__ void trsm_block(int n, S* __restrict__ A, int lda, int J, int w, cons
       const int k = 4 * q + lk;
       x2[q] = (rv && k < w) ? A[row + (size_t)(J + k) * lda] : S(0);
     }
+    S part[4] = {S(0), S(0), S(0), S(0)};
 #pragma unroll
     for (int ct = 0; ct < CT; ct++) {
       typename M::acc_t acc = {0, 0, 0, 0};
@@ -318,29 +328,77 @@ __device__ void trsm_block(int n, S* __restrict__ A, int lda, int J, int w, cons
       for (int r = 0; r < 4; r++) {
         const int col = 16 * ct + M::row(lane, r);
         if (rv && col < w) A[row + (size_t)(J + col) * lda] = acc[r];
+        if (Y && col < w) {
+#pragma unroll
+          for (int q = 0; q < 4; q++) part[q] += acc[r] * Y[(J + col) * 4 + q];
+        }
       }
+    }
+    if (Y) {
+#pragma unroll
+      for (int q = 0; q < 4; q++) {
+        part[q] += __shfl_xor(part[q], 16, 64);
+        part[q] += __shfl_xor(part[q], 32, 64);
+      }
+      const S mine = lk == 0 ? part[0] : (lk == 1 ? part[1] : (lk == 2 ? part[2] : part[3]));
+      if (rv) Y[row * 4 + lk] -= mine;
     }
   }
 }
 
+// Whether a factorization can carry the forward solve of nrhs right-hand
+// sides in LDS.
+__device__ __forceinline__ bool fuse_fits(int n, int nrhs) {
+  return nrhs > 0 && nrhs <= 4 && 4 * n <= kYmax;
+}
+
 // Factors one system; returns 0 or the LAPACK info (first failing minor).
+// With B (fuse_fits(n, nrhs)): also solves L Y = B, by the right-looking
+// forward substitution in LDS (y_J = inv(L_JJ) b_J, then the block column's
+// TRSM subtracts L[J+w:n, J] y_J), and on success writes Y over B; B is left
+// as it was when the factorization fails.
 template <class S, int T>
-__device__ int chol_sys(int n, S* __restrict__ A, int lda, Smem& sm) {
+__device__ int chol_sys(int n, S* __restrict__ A, int lda, Smem& sm, S* __restrict__ B = nullptr,
+                        int ldb = 0, int nrhs = 0) {
   S* D = reinterpret_cast<S*>(sm.D);
   S* V = reinterpret_cast<S*>(sm.V);
+  S* Y = B ? reinterpret_cast<S*>(sm.Y) : nullptr;
+  const int t = threadIdx.x;
+  if (Y) {
+    for (int e = t; e < 4 * n; e += kBT) {
+      const int i = e >> 2, q = e & 3;
+      Y[e] = q < nrhs ? B[i + (size_t)q * ldb] : S(0);
+    }
+  }
   for (int J = 0; J < n; J += T) {
     const int w = min(T, n - J);
     if (J > 0) update_block<S, T>(n, A, lda, J, w, D);  // stages in D and V's space
     __syncthreads();
-    if (threadIdx.x < 64) {
+    if (t < 64) {
       const int f = factor_diag<S, T>(A, lda, J, w, V, D);  // D: the broadcast line
-      if (threadIdx.x == 0) sm.flag = f ? J + f : 0;
+      if (t == 0) sm.flag = f ? J + f : 0;
     }
     __syncthreads();
     const int f = sm.flag;
     if (f) return f;
     lower_inverse_inplace<S, kTmax>(V, w);  // V <- inv(L_JJ), all threads
-    trsm_block<S, T>(n, A, lda, J, w, V);
+    if (Y) {  // y_J = inv(L_JJ) b_J in place: every read, a barrier, the writes
+      const int i = t >> 2, q = t & 3;
+      S y = S(0);
+      if (i < w)
+        for (int c = 0; c <= i; c++) y += V[i * kDP + c] * Y[(J + c) * 4 + q];
+      __syncthreads();
+      if (i < w) Y[(J + i) * 4 + q] = y;
+      __syncthreads();
+    }
+    trsm_block<S, T>(n, A, lda, J, w, V, Y);
+    __syncthreads();
+  }
+  if (Y) {
+    for (int e = t; e < 4 * n; e += kBT) {
+      const int i = e >> 2, q = e & 3;
+      if (q < nrhs) B[i + (size_t)q * ldb] = Y[e];
+    }
     __syncthreads();
   }
   return 0;
@@ -475,15 +533,16 @@ __device__ void offdiag_update(const S* __restrict__ L, int lda, int J, int w, i
   }
 }
 
-// Solves L L^T X = B in place (B: n x nrhs, column-major, ldb).
+// Solves L L^T X = B in place (B: n x nrhs, column-major, ldb); with fwd
+// false B already holds Y = L^-1 B and only L^T X = Y is solved.
 template <class S, int T>
 __device__ void solve_sys(int n, const S* __restrict__ L, int lda, S* __restrict__ B, int ldb,
-                          int nrhs, Smem& sm) {
+                          int nrhs, Smem& sm, bool fwd = true) {
   if (n <= 0) return;
   S* BS = reinterpret_cast<S*>(sm.bs);  // solved block, [c][q] for q < 4
   const int t = threadIdx.x;
   // forward: L Y = B
-  for (int J = 0; J < n; J += T) {
+  for (int J = 0; fwd && J < n; J += T) {
     const int w = min(T, n - J);
     if (t < 64) diag_solve<S, T, false>(L, lda, J, w, B, ldb, nrhs, BS, sm);
     __syncthreads();
@@ -502,14 +561,16 @@ __device__ void solve_sys(int n, const S* __restrict__ L, int lda, S* __restrict
 
 // width: the system's block-column width (16 or 32), chosen on the host
 template <class S>
-__device__ int chol_any(int n, S* A, int lda, int width, Smem& sm) {
-  return (width == 16) ? chol_sys<S, 16>(n, A, lda, sm) : chol_sys<S, 32>(n, A, lda, sm);
+__device__ int chol_any(int n, S* A, int lda, int width, Smem& sm, S* B = nullptr, int ldb = 0,
+                        int nrhs = 0) {
+  return (width == 16) ? chol_sys<S, 16>(n, A, lda, sm, B, ldb, nrhs)
+                       : chol_sys<S, 32>(n, A, lda, sm, B, ldb, nrhs);
 }
 template <class S>
 __device__ void solve_any(int n, const S* L, int lda, S* B, int ldb, int nrhs, int width,
-                          Smem& sm) {
-  if (width == 16) solve_sys<S, 16>(n, L, lda, B, ldb, nrhs, sm);
-  else solve_sys<S, 32>(n, L, lda, B, ldb, nrhs, sm);
+                          Smem& sm, bool fwd = true) {
+  if (width == 16) solve_sys<S, 16>(n, L, lda, B, ldb, nrhs, sm, fwd);
+  else solve_sys<S, 32>(n, L, lda, B, ldb, nrhs, sm, fwd);
 }
 
 // Takes the next queue entry (or -1 when drained); every thread gets it.
@@ -552,9 +613,14 @@ posv_batched_kernel(int nsys, const SysDesc<S>* __restrict__ desc, int nrhs, int
           d.B[i + (size_t)r * d.ldb] = d.Bsrc[i + (size_t)r * d.ldb];
       __syncthreads();
     }
+    // factor + solve: the forward substitution runs inside the factorization
+    // when the right-hand sides fit in LDS
+    const bool fuse = mode == 1 && fuse_fits(d.n, nrhs);
     int f = 0;
-    if (mode != 2) f = chol_any<S>(d.n, d.A, d.lda, d.width, sm);
-    if (f == 0 && mode != 0) solve_any<S>(d.n, d.A, d.lda, d.B, d.ldb, nrhs, d.width, sm);
+    if (mode != 2)
+      f = fuse ? chol_any<S>(d.n, d.A, d.lda, d.width, sm, d.B, d.ldb, nrhs)
+               : chol_any<S>(d.n, d.A, d.lda, d.width, sm);
+    if (f == 0 && mode != 0) solve_any<S>(d.n, d.A, d.lda, d.B, d.ldb, nrhs, d.width, sm, !fuse);
     if (threadIdx.x == 0 && mode != 2) info[d.idx] = f;
   }
 }
@@ -683,15 +749,20 @@ mixed_posv_kernel(int nsys, const MixDesc* __restrict__ desc, int nrhs, int iter
       it = 0;
     } else if (ov != 0.0) {
       it = -2;
-    } else if (chol_any<float>(n, d.A32, n, d.width, sm) != 0) {
-      it = -3;
     } else {
-      // X = float solve of B
+      // W32 = float(B); the factorization runs its forward solve alongside
+      // when the right-hand sides fit in LDS
       for (int r = 0; r < nrhs; r++)
         for (int i = threadIdx.x; i < n; i += kBT)
           d.W32[i + (size_t)r * n] = (float)d.B[i + (size_t)r * d.ldb];
       __syncthreads();
-      solve_any<float>(n, d.A32, n, d.W32, n, nrhs, d.width, sm);
+      const bool fuse = fuse_fits(n, nrhs);
+      it = (fuse ? chol_any<float>(n, d.A32, n, d.width, sm, d.W32, n, nrhs)
+                 : chol_any<float>(n, d.A32, n, d.width, sm)) != 0 ? -3 : 0;
+    }
+    if (it == 0 && n > 0) {
+      // X = float solve of B
+      solve_any<float>(n, d.A32, n, d.W32, n, nrhs, d.width, sm, !fuse_fits(n, nrhs));
       for (int r = 0; r < nrhs; r++)
         for (int i = threadIdx.x; i < n; i += kBT)
           d.X[i + (size_t)r * d.ldx] = (double)d.W32[i + (size_t)r * n];
