@@ -190,6 +190,50 @@ __device__ inline void tri_tile_of(int id, int mode, int tilesM, int tilesN, int
   }
 }
 
+// The same tiles in groups of ATLG_GEMM_GM tile rows (mode 1; tile columns
+// for mode 2), each group column by column, like tile_of's groups: the
+// workgroups resident on one XCD then cover a few tile rows by a few tile
+// columns (a few A and B panels in its L2) instead of one or two whole rows
+// of the triangle (as many B panels as the row is long).
+#ifndef ATLG_GEMM_TRI_GROUPED
+#define ATLG_GEMM_TRI_GROUPED 1
+#endif
+__device__ inline void tri_tile_grouped(int id, int mode, int tilesM, int tilesN, int& tm,
+                                        int& tn) {
+  constexpr int GM = ATLG_GEMM_GM;
+  const int R = mode == 1 ? tilesM : tilesN, Cn = mode == 1 ? tilesN : tilesM;
+  for (int r0 = 0;; r0 += GM) {
+    const int r1 = min(r0 + GM, R), h = r1 - r0;
+    const int full = min(r0, Cn);                  // columns with all h rows
+    const int tc = max(0, min(r1, Cn) - r0);       // then column r0 + j has h - j rows
+    const int cnt = full * h + tc * h - tc * (tc - 1) / 2;
+    if (id < cnt || r1 >= R) {
+      int a, b;
+      if (id < full * h) {
+        b = id / h;
+        a = r0 + id % h;
+      } else {
+        int o = id - full * h, j = 0;
+        while (j + 1 < tc && o >= h - j) {
+          o -= h - j;
+          j++;
+        }
+        b = r0 + j;
+        a = r0 + j + o;
+      }
+      if (mode == 1) {
+        tm = a;
+        tn = b;
+      } else {
+        tn = a;
+        tm = b;
+      }
+      return;
+    }
+    id -= cnt;
+  }
+}
+
 template <class S, bool AT, bool BT, bool VA, bool VB>
 __global__ void __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(2, 2)))
 gemm_mfma(int M, int N, int K, S alpha, const S* A, long lda, const S* B, long ldb, S beta,
@@ -214,7 +258,8 @@ gemm_mfma(int M, int N, int K, S alpha, const S* A, long lda, const S* B, long l
     const int id = !ATLG_GEMM_XCD ? bid
                    : (xcd < rem) ? xcd * (per + 1) + local
                                  : rem * (per + 1) + (xcd - rem) * per + local;
-    tri_tile_of(id, mode, tilesM, tilesN, tm, tn);
+    if (ATLG_GEMM_TRI_GROUPED) tri_tile_grouped(id, mode, tilesM, tilesN, tm, tn);
+    else tri_tile_of(id, mode, tilesM, tilesN, tm, tn);
   }
   const int m0 = tm * BM, n0 = tn * BN;
 
@@ -357,7 +402,8 @@ __device__ __forceinline__ void gemm_rega_f64(int M, int N, int K, double alpha,
       const int id = (!ATLG_GEMM_XCD || !compact) ? bid
                      : (xcd < rem) ? xcd * (per + 1) + local
                                    : rem * (per + 1) + (xcd - rem) * per + local;
-      tri_tile_of(id, mode, tilesM, tilesN, tm, tn);
+      if (ATLG_GEMM_TRI_GROUPED && compact) tri_tile_grouped(id, mode, tilesM, tilesN, tm, tn);
+      else tri_tile_of(id, mode, tilesM, tilesN, tm, tn);
     }
     const int m0 = tm * BM, n0 = tn * BN;
     // K range of the tile: all of K, or with a triangular operand whose
