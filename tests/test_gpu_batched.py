@@ -192,6 +192,48 @@ def test_dpotrf_batched_info(G):
             assert llt_ratio(mats[k], b.mat(k, host), EPS64) <= 16.0
 
 
+def test_dposv_batched_info_keeps_rhs(G):
+    """posv with non-SPD systems mixed in, on both sides of the order (512)
+    up to which the forward solve runs inside the factorization: info equals
+    the oracle potf2's, a failing system's right-hand sides (and every
+    system's padding rows of B) are left as they were, the others solve."""
+    import torch
+    rng = np.random.default_rng(23)
+    nrhs = 3
+    ns = [5, 40, 129, 300, 511, 512, 513, 600, 64, 200, 520, 17]
+    mats, want, rhs = [], [], []
+    for k, n in enumerate(ns):
+        a = spd(n, rng)
+        if k % 3 == 1:  # a non-positive pivot part way down
+            p = int(rng.integers(n // 2, n))
+            a[p, p] = -1.0 - a[p, p]
+        mats.append(a)
+        want.append(O.potf2(O.Lower, np.asfortranarray(a.copy())))
+        rhs.append(rng.standard_normal((n, nrhs)))
+    ldbs = [n + 2 for n in ns]
+    b = Batch(mats, ns, rhs, ldbs)
+    info = torch.full((len(ns),), -7, dtype=torch.int32, device="cuda")
+    G.dposv_batched(b.n, b.ptrs, b.lda, b.bptrs, b.ldb, nrhs, info)
+    torch.cuda.synchronize()
+    got = info.cpu().numpy()
+    host, bhost = b.buf.cpu().numpy(), b.bbuf.cpu().numpy()
+    assert list(got) == want
+    assert any(w > 0 for w in want) and any(w == 0 for w in want)
+    for k, n in enumerate(ns):
+        check_upper_untouched(b, k, host)
+        o, ld = b.boffs[k], b.ldb[k]
+        now = bhost[o:o + ld * nrhs].reshape(nrhs, ld).T
+        was = b.bhost0[o:o + ld * nrhs].reshape(nrhs, ld).T
+        assert np.array_equal(now[n:], was[n:]), "padding rows of B, system %d" % k
+        if want[k] > 0:
+            assert np.array_equal(now, was), "B of failing system %d changed" % k
+            continue
+        xc = oracle_solve(mats[k], rhs[k])
+        r = O.slv_resid(n, nrhs, O.genrm1(np.asfortranarray(mats[k])), xc,
+                        np.asfortranarray(b.rhs(k, nrhs, bhost)))
+        assert r <= 100.0, (k, n, r)
+
+
 def test_dpotrs_batched_with_given_factor(G):
     """Solve-only entry with factors from the oracle."""
     import torch
